@@ -1,0 +1,214 @@
+"""Pins the CPU oracle (oracle/se_oracle.c) before anything is compared with it.
+
+Checks, per function: the reference's own outputs (golden vectors from the compiled
+SE_fgg.c), closed forms, the DFT definition, two independent forms of the direct
+Ewald sums, finite differences, and SE-vs-direct agreement at the FGG error floor.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_files, load_golden, rms_rel
+
+G3 = golden_files("g3p")
+G2 = golden_files("g2p")
+G1 = golden_files("g1p")
+
+
+def _params(O, g):
+    dims = tuple(int(v) for v in g["dims"])
+    return O.pack_params(int(g["N"]), dims, int(g["P"]), float(g["c"]), float(g["h"]),
+                         float(g["a"]), float(g["b"]))
+
+
+def _close(a, b, tol=1e-13):
+    scale = max(np.max(np.abs(b)), 1e-300)
+    assert np.max(np.abs(np.asarray(a) - np.asarray(b))) <= tol * scale
+
+
+@pytest.mark.parametrize("path", G3 + G2 + G1, ids=lambda p: p.split("/")[-1][:-4])
+def test_oracle_matches_reference_golden(oracle, path):
+    """Expansion, spread, wrap, extend, interpolation vs the compiled reference."""
+    O = oracle
+    g = load_golden(path)
+    per = int(g["per"])
+    p = _params(O, g)
+    zs = O.base_gaussian(p)
+    _close(zs, g["zs"], 1e-14)
+    ex = O.expand_all(per, g["x"], p)
+    assert np.array_equal(ex["idx"], g["idx"])
+    for k in ("zx", "zy", "zz"):
+        _close(ex[k], g[k], 1e-13)
+    H = O.grid_split(p, g["q"], ex, zs)
+    if "H_split" in g:
+        _close(H, g["H_split"], 1e-13)
+        _close(O.grid(per, p, g["x"], g["q"], zs), g["H_grid"], 1e-13)
+    Hper = np.zeros(len(g["Hper"]))
+    O.wrap(per, p, H, Hper)
+    _close(Hper, g["Hper"], 1e-13)
+    Hext = O.extend(per, p, g["G"])
+    if "H_ext" in g:
+        _close(Hext, g["H_ext"], 0.0)
+    phi = O.int_split(p, Hext, ex, zs)
+    _close(phi, g["phi_split"], 1e-12)
+    _close(phi, g["phi_int"], 1e-12)
+    if "zfx" in g:
+        exf = O.expand_all(per, g["x"], p, force=True)
+        assert np.array_equal(exf["idx"], g["fidx"])
+        for k, gk in (("zx", "fzx"), ("zy", "fzy"), ("zz", "fzz"), ("zfx", "zfx"), ("zfy", "zfy"), ("zfz", "zfz")):
+            _close(exf[k], g[gk], 1e-13)
+        if "F_split" in g:
+            F = O.int_split_force(p, Hext, g["q"], exf, zs)
+            _close(F, g["F_split"], 1e-12)
+            _close(F, g["F_int"], 1e-12)
+
+
+@pytest.mark.parametrize("P", [3, 4, 5, 8, 11, 16, 21])
+def test_expansion_closed_form(oracle, P):
+    """zx_i zy_j zz_k zs_ijk == q d exp(-c |x - g_ijk|^2), g on the periodic grid image."""
+    O = oracle
+    rng = np.random.default_rng(P)
+    M, N = 12, 6
+    h = 1.0 / M
+    c = 40.0 + P
+    p = O.pack_params(N, M, P, c, h)
+    x = rng.random(3 * N)
+    ex = O.expand_all(O.P3, x, p)
+    zs = O.base_gaussian(p).reshape(P, P, P)
+    ph = O.half(P)
+    first = -ph if P & 1 else -ph + 1
+    for n in range(N):
+        xn = x[n::N][:3] if False else np.array([x[n], x[n + N], x[n + 2 * N]])
+        iota = np.round(xn / h) if P & 1 else np.floor(xn / h)
+        g = h * (iota[:, None] + first + np.arange(P)[None, :])       # 3 x P grid coords
+        W = (ex["zx"][n * P:(n + 1) * P][:, None, None] * ex["zy"][n * P:(n + 1) * P][None, :, None]
+             * ex["zz"][n * P:(n + 1) * P][None, None, :] * zs)
+        r2 = ((xn[0] - g[0])[:, None, None] ** 2 + (xn[1] - g[1])[None, :, None] ** 2
+              + (xn[2] - g[2])[None, None, :] ** 2)
+        want = p.d * np.exp(-c * r2)
+        assert np.max(np.abs(W - want)) <= 1e-12 * want.max()
+        # first support point sits at extended index (iota + first + P_half)
+        e = (iota + first + ph).astype(int)
+        assert ex["idx"][n] == (e[0] * p.npdims[1] + e[1]) * p.npdims[2] + e[2]
+
+
+def test_spread_brute_force_periodic(oracle):
+    """wrap(spread(x)) equals the brute-force sum of truncated periodic Gaussians."""
+    O = oracle
+    rng = np.random.default_rng(5)
+    M, N, P = 10, 5, 5
+    h = 1.0 / M
+    c = 60.0
+    p = O.pack_params(N, M, P, c, h)
+    x = rng.random(3 * N)
+    q = rng.uniform(-1, 1, N)
+    zs = O.base_gaussian(p)
+    Hper = O.wrap(O.P3, p, O.grid_split(p, q, O.expand_all(O.P3, x, p), zs)).reshape(M, M, M, order="F")
+    want = np.zeros((M, M, M))
+    ph = O.half(P)
+    for n in range(N):
+        xn = np.array([x[n], x[n + N], x[n + 2 * N]])
+        iota = np.round(xn / h).astype(int)
+        for i in range(-ph, ph + 1):
+            for j in range(-ph, ph + 1):
+                for k in range(-ph, ph + 1):
+                    gi = iota + np.array([i, j, k])
+                    r2 = np.sum((xn - h * gi) ** 2)
+                    want[gi[0] % M, gi[1] % M, gi[2] % M] += q[n] * p.d * np.exp(-c * r2)
+    assert np.max(np.abs(Hper - want)) <= 1e-12 * np.abs(want).max()
+
+
+def test_fft_matches_dft_definition_and_numpy(oracle):
+    O = oracle
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((3, 4, 5)) + 1j * rng.standard_normal((3, 4, 5))
+    n = np.indices(a.shape).reshape(3, -1).T
+    dft = np.zeros(a.size, dtype=complex)
+    for idx, kk in enumerate(n):
+        ph = np.exp(-2j * np.pi * (n[:, 0] * kk[0] / 3 + n[:, 1] * kk[1] / 4 + n[:, 2] * kk[2] / 5))
+        dft[idx] = np.sum(a.reshape(-1) * ph)
+    assert np.allclose(O.fft3d(a, -1).reshape(-1), dft, rtol=0, atol=1e-12)
+    b = rng.standard_normal((12, 8, 18)) + 0j
+    assert np.allclose(O.fft3d(b, -1), np.fft.fftn(b), rtol=0, atol=1e-11)
+    assert np.allclose(O.fft3d(b, +1), np.fft.ifftn(b) * b.size, rtol=0, atol=1e-11)
+
+
+def test_kvec_and_scaling_closed_form(oracle):
+    O = oracle
+    L = 2.5
+    k = O.kvec(8, L)
+    assert np.allclose(k, 2 * np.pi / L * np.array([0, 1, 2, 3, -4, -3, -2, -1]))
+    k5 = O.kvec(5, L)
+    assert np.allclose(k5, 2 * np.pi / L * np.array([0, 1, 2, -2, -1]))
+    M, s = 6, -0.01
+    Z = O.scaling(s, M, L).reshape(M, M, M)
+    kk = O.kvec(M, L)
+    i, j, l = 1, 4, 5
+    K2 = kk[i] ** 2 + kk[j] ** 2 + kk[l] ** 2
+    assert np.isclose(Z[i, j, l], np.exp(s * K2) / K2 * 4 * np.pi / M ** 3, rtol=1e-14)
+
+
+def test_direct3p_two_forms_agree(oracle):
+    """Reference's pairwise hs-weighted cosine sum == structure-factor sum."""
+    O = oracle
+    rng = np.random.default_rng(2)
+    N = 12
+    x = rng.random(3 * N) * 1.3
+    q = rng.choice([-1.0, 1.0], N)
+    box = [1.3, 1.3, 1.3]
+    a = O.direct3p(x, q, box, 4.0, 5, pairwise=True)
+    b = O.direct3p(x, q, box, 4.0, 5)
+    assert rms_rel(b, a) < 1e-13
+    fa = O.direct3p(x, q, box, 4.0, 4, force=True, pairwise=True)
+    fb = O.direct3p(x, q, box, 4.0, 4, force=True)
+    assert rms_rel(fb, fa) < 1e-13
+
+
+def test_direct3p_force_is_half_q_grad_phi(oracle):
+    """F_m = (q_m/2) grad phi(x_m)  (SE_direct.c:445-459 convention), by finite differences."""
+    O = oracle
+    rng = np.random.default_rng(3)
+    N, L, xi, K = 10, 1.0, 3.0, 6
+    x = rng.random((3, N))
+    q = rng.uniform(-1, 1, N)
+    F = O.direct3p(x.reshape(-1), q, L, xi, K, force=True).reshape(3, N)
+    eps = 1e-5
+    for m in (0, 7):
+        for d in range(3):
+            vals = []
+            for s in (+1, -1):
+                y = x[:, m].copy()
+                y[d] += s * eps
+                xa = np.concatenate([y[:, None], x], axis=1).reshape(-1)   # probe as a zero charge
+                qa = np.concatenate([[0.0], q])
+                vals.append(O.direct3p(xa, qa, L, xi, K, n_targets=1)[0])
+            grad = (vals[0] - vals[1]) / (2 * eps)
+            assert abs(F[d, m] - 0.5 * q[m] * grad) < 1e-6 * max(1.0, abs(F[d, m]))
+
+
+def test_se3p_oracle_vs_direct_config1(oracle):
+    """N=100 unit cube, q=+-1, P=21, M=32, xi=6: SE vs direct k-space sum (|j| <= 15)."""
+    O = oracle
+    rng = np.random.default_rng(1)
+    N, P, M, xi = 100, 21, 32, 6.0
+    x = rng.random(3 * N)
+    q = np.array([1.0, -1.0] * (N // 2))
+    rng.shuffle(q)
+    m = 0.95 * np.sqrt(np.pi * P)
+    phi = O.se3p(x, q, 1.0, M, P, m, xi)
+    ref = O.direct3p(x, q, 1.0, xi, 15)
+    assert rms_rel(phi, ref) < 1e-11
+    F = O.se3p(x, q, 1.0, M, P, m, xi, force=True)
+    Fref = O.direct3p(x, q, 1.0, xi, 15, force=True)
+    assert rms_rel(F, Fref) < 1e-10
+
+
+def test_init_params_rule(oracle):
+    """SE_general.c:517-572 with the reference test's values (L=3.1814, xi=8)."""
+    O = oracle
+    g = load_golden(golden_files("atoms")[0]) if golden_files("atoms") else None
+    import os
+    from conftest import GOLDEN
+    a = np.load(os.path.join(GOLDEN, "atoms.npz"))["xyzq"]
+    M, m, Q = O.init_params(21, 3.1814, 8.0, a[:100, 3])
+    assert M == 108 and np.isclose(m, 0.95 * np.sqrt(21 * np.pi))
+    assert np.isclose(Q, np.sum(a[:100, 3] ** 2))
