@@ -1,0 +1,219 @@
+"""SE3P spread+interp throughput on one MI355X.
+
+Workload (default): N = 1,000,000 charges uniform in the unit cube, charges uniform in
+[-1, 1] shifted to zero sum, P = 16, M = 128^3, xi = 40, FP32 spread / FFT / gather,
+potential and force.  One step = one full SE3P k-space solve with x, q already in HBM:
+bucketing + spread + halo merge, R2C FFT + Green's function + C2R FFT, gather (phi, F).
+
+value = 2 N P^3 / (t_spread_stage + t_interp_stage)   [particle*P^3 updates/s]
+(the reference metric counts expand, spread, wrap, extend and interpolation and
+excludes the FFT; the full-solve rate is reported beside it).
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector, packed; equal to its FP32 matrix peak)
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def make_system(N, seed=2):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    x = rng.random(3 * N)
+    q = rng.uniform(-1.0, 1.0, N)
+    q -= q.mean()
+    return x, q
+
+
+def cpu_baseline(M, P, xi, n_sample, seed=2):
+    """The reference's own gridding core (oracle/_ref, SE_fgg.c built with its -mavx
+    flags) on one host core: expand (force factors) + AVX spread + wrap + extend + AVX
+    force/potential interpolation on a bounded sample of the same workload."""
+    import ctypes as C
+
+    import numpy as np
+
+    import oracle as O
+    if not O.ref_available(O.P3):
+        return None
+    L = O.ref_lib(O.P3)
+    x, q = make_system(n_sample, seed)
+    m = 0.95 * np.sqrt(np.pi * P)
+    h, w, eta, c = O.parse_params(1.0, M, P, m, xi)
+    p = O.SE_FGG_params()
+    L.SE_FGG_pack_params(C.byref(p), n_sample, M, M, M, P, c, h)
+    st, keep = O.ref_state(x, q)
+    wk = O.RefWork(p, force=True)
+    Hper = O.aligned_zeros(M ** 3)
+    F = O.aligned_zeros(3 * n_sample)
+    L.SE_FGG_base_gaussian(C.byref(wk.w), C.byref(p))
+    t = {}
+    t0 = time.perf_counter()
+    L.SE_FGG_expand_all_SSE_force(C.byref(wk.w), C.byref(st), C.byref(p))
+    t1 = time.perf_counter()
+    L.SE_FGG_grid_split_AVX_dispatch_force(C.byref(wk.w), C.byref(st), C.byref(p))
+    t2 = time.perf_counter()
+    L.SE_FGG_wrap_fcn(Hper, C.byref(wk.w), C.byref(p))
+    t3 = time.perf_counter()
+    L.SE_FGG_extend_fcn(C.byref(wk.w), Hper, C.byref(p))
+    t4 = time.perf_counter()
+    L.SE_FGG_int_split_AVX_dispatch_force(F, C.byref(st), C.byref(wk.w), C.byref(p))
+    t5 = time.perf_counter()
+    t.update(expand=t1 - t0, spread=t2 - t1, wrap=t3 - t2, extend=t4 - t3, interp=t5 - t4)
+    total = t5 - t0
+    return {
+        "value": 2.0 * n_sample * P ** 3 / total,
+        "unit": "particle*P^3 updates/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": "N=%d of the same uniform system, P=%d, M=%d^3, FP64 force path: expand_all_SSE_force + "
+                  "grid_split_AVX_dispatch_force + wrap + extend + int_split_AVX_dispatch_force (%.1f s)"
+                  % (n_sample, P, M, total),
+        "stage_s": {k: round(v, 4) for k, v in t.items()},
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+    }
+
+
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run(args):
+    import numpy as np
+    import torch
+
+    import se_fast_gridding_amd as se
+
+    dev = torch.device("cuda:0")
+    N, M, P, xi = args.n, args.m, args.p, args.xi
+    x, q = make_system(N)
+    xt = torch.tensor(x, device=dev)
+    qt = torch.tensor(q, device=dev)
+    phi = torch.empty(N, dtype=torch.float64, device=dev)
+    F = torch.empty(3 * N, dtype=torch.float64, device=dev)
+    prec = se.PREC_FP32 if args.precision == "fp32" else se.PREC_FP64
+    stream = torch.cuda.current_stream()
+    plan = se.Plan(M, P, 1.0, xi, N, precision=prec, stream=stream)
+
+    for _ in range(args.warmup):
+        plan.solve(xt, qt, phi, F)
+    torch.cuda.synchronize()
+
+    # (1) wall clock over K full solves
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.solve(xt, qt, phi, F)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+
+    # (2) HIP events on the plan's stream around each stage / kernel
+    plan.enable_timing(True)
+    acc = {}
+    for _ in range(args.steps):
+        plan.solve(xt, qt, phi, F)
+        for k, v in plan.timings().items():
+            acc[k] = acc.get(k, 0.0) + v
+    plan.enable_timing(False)
+    ms = {k: v / args.steps for k, v in acc.items()}
+    info = plan.info()
+    upd = 2.0 * N * P ** 3
+    t_si = (ms["spread_stage"] + ms["interp_stage"]) * 1e-3
+    value = upd / t_si
+
+    # dominant kernel roofline (compute-bound: one FMA per update; gather with force: two)
+    peak = FP32_PEAK_TFLOPS if prec == se.PREC_FP32 else FP64_PEAK_TFLOPS
+    spread_flops = 2.0 * N * P ** 3
+    gather_flops = 4.0 * N * P ** 3
+    if ms["spread_kernel"] >= ms["gather_kernel"]:
+        kname, kflops, kms = "k_spread_tiles", spread_flops, ms["spread_kernel"]
+    else:
+        kname, kflops, kms = "k_interp_tiles", gather_flops, ms["gather_kernel"]
+    achieved = kflops / (kms * 1e-3) / 1e12
+
+    # accuracy: FP32 P=16 vs FP64 P=21 on the same system (FGG floor ~1e-13 at P=21)
+    err = {}
+    if args.check and prec == se.PREC_FP32:
+        phi64 = torch.empty_like(phi)
+        F64 = torch.empty_like(F)
+        ref = se.Plan(M, 21, 1.0, xi, N, precision=se.PREC_FP64, stream=stream)
+        ref.enable_timing(True)
+        ref.solve(xt, qt, phi64, F64)
+        t64 = ref.timings()
+        torch.cuda.synchronize()
+        ref.close()
+        rr = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))  # noqa: E731
+        err = {"phi_rms_rel_vs_fp64_p21": rr(phi, phi64), "force_rms_rel_vs_fp64_p21": rr(F, F64),
+               "fp64_p21_updates_per_s": 2.0 * N * 21 ** 3 / ((t64["spread_stage"] + t64["interp_stage"]) * 1e-3),
+               "fp64_p21_stage_ms": {k: round(v, 4) for k, v in t64.items()}}
+    plan.close()
+
+    cpu = None
+    if args.cpu_sample > 0:
+        cpu = cpu_baseline(M, P, xi, args.cpu_sample)
+
+    out = {
+        "metric": "spread+interp particle*P^3 updates/s",
+        "value": value,
+        "unit": "particle*P^3 updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == se.PREC_FP32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": "SE3P potential+force, single system: N=%d uniform charges in L=1, zero net "
+                               "charge, P=%d, M=%d^3, xi=%g" % (N, P, M, xi),
+                   "N": N, "P": P, "M": M, "xi": xi, "precision": args.precision, "quantities": "phi+force",
+                   "tile": info["tile"], "chunk": info["chunk"], "block": info["block"], "work_items": info["items"]},
+        "solve_ms": wall * 1e3,
+        "stage_ms": {k: round(v, 4) for k, v in ms.items()},
+        "roofline": {"bound": "mfma", "compute": "VALU FP32 FMA (MFMA not used); peak is MI355X FP32",
+                     "kernel": kname, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "flops_per_launch": kflops, "kernel_ms": kms, "traffic": None},
+        "accuracy": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--m", type=int, default=128)
+    ap.add_argument("--p", type=int, default=16)
+    ap.add_argument("--xi", type=float, default=40.0)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--no-check", dest="check", action="store_false")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has exactly one; "
+              "multi-GPU sharding is a design note, not a measured path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    run(args)
+
+
+if __name__ == "__main__":
+    main()

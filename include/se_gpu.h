@@ -1,0 +1,97 @@
+/*
+ * se_gpu.h -- device-resident Spectral Ewald engine of libse_fgg_amd.so.
+ *
+ * No reference counterpart: the reference has no device boundary.  This is the
+ * interface the fast path runs behind.  A plan owns one grid geometry, its
+ * rocFFT plans and all scratch in HBM; particle arrays are DEVICE pointers in the
+ * reference's layout (x SoA: x[0..N) y[N..2N) z[2N..3N); q[N]; phi[N]; F SoA[3N])
+ * and every call is asynchronous on the plan's stream.  Functions return 0 or a
+ * negative SE_ERR_* code; se_gpu_last_error() returns the last message.
+ *
+ * The solve is the reference driver (spectral_ewald.c:83-165) re-staged for the GPU:
+ *   se_spread  = expand + spread + wrap   (particles bucketed by grid tile, P^3
+ *                factors built in registers, LDS sub-grids, fixed-order halo merge)
+ *   se_kspace  = FFT, Green's-function scaling, inverse FFT (rocFFT R2C/C2R)
+ *   se_interp  = extend + interpolate     (LDS-staged P^3 gather per particle)
+ */
+#ifndef SE_FGG_AMD_SE_GPU_H
+#define SE_FGG_AMD_SE_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SE_OK 0
+#define SE_ERR_ARG -1
+#define SE_ERR_HIP -2
+#define SE_ERR_FFT -3
+#define SE_ERR_CAPACITY -4
+#define SE_ERR_RANGE -5
+
+#define SE_PREC_FP64 0
+#define SE_PREC_FP32 1
+
+#define SE_Q_POTENTIAL 1
+#define SE_Q_FORCE 2
+
+typedef struct se_plan se_plan;
+
+typedef struct {
+    int periodicity;   /* 3 (SE3P); 2P/1P gridding primitives are in se_fgg.h */
+    int M;             /* grid points per side of the cubic periodic grid */
+    int P;             /* Gaussian support, 1..32 */
+    double box;        /* side length L of the cubic box */
+    double xi;         /* Ewald parameter */
+    double m;          /* Gaussian shape; 0 selects the driver's 0.95*sqrt(pi*P) */
+    int precision;     /* SE_PREC_FP64 or SE_PREC_FP32 (spread, FFT and gather arithmetic) */
+    int n_max;         /* particle capacity */
+    int tile;          /* tile side in grid cells, 0 = automatic */
+    int chunk;         /* max particles per work item, 0 = automatic */
+    int block;         /* threads per work-group for spread/gather, 0 = automatic */
+} se_plan_desc;
+
+int se_plan_create(se_plan** plan, const se_plan_desc* desc, void* hip_stream);
+int se_plan_destroy(se_plan* plan);
+int se_plan_set_stream(se_plan* plan, void* hip_stream);
+/* Bytes of HBM the plan holds (for admission control). */
+size_t se_plan_bytes(const se_plan* plan);
+/* Bytes a plan with this description would hold, without creating it. */
+size_t se_plan_bytes_for(const se_plan_desc* desc);
+
+/* Full SE3P k-space solve.  phi and/or force may be NULL (not computed). */
+int se_solve(se_plan* plan, int N, const double* d_x, const double* d_q, double* d_phi, double* d_force);
+
+/* Stages of se_solve, for timing and for callers that bring their own grid. */
+int se_spread(se_plan* plan, int N, const double* d_x, const double* d_q);
+int se_kspace(se_plan* plan);
+int se_interp(se_plan* plan, double* d_phi, double* d_force);
+
+/* Periodic grid of the plan (row-major, z fastest; float or double per precision). */
+int se_plan_grid(se_plan* plan, void** d_grid, int* elem_bytes);
+
+/*
+ * Instrumentation: when enabled, se_solve/se_spread/se_interp record HIP events
+ * on the plan's stream around the spread kernel, the merge kernel, the gather
+ * kernel, the k-space stage and the whole spread/interp stages.  se_plan_timings
+ * synchronises the stream and returns milliseconds of the last call:
+ * ms[0]=spread stage (bucketing+spread+merge), ms[1]=spread kernel, ms[2]=merge,
+ * ms[3]=k-space, ms[4]=interp stage, ms[5]=gather kernel, ms[6]=bucketing.
+ */
+int se_plan_enable_timing(se_plan* plan, int on);
+int se_plan_timings(se_plan* plan, float ms[8]);
+
+/* Tile/work-item geometry chosen for the plan: out[0]=tile, [1]=chunk, [2]=block,
+ * [3]=number of tiles, [4]=work items of the last spread (syncs). */
+int se_plan_info(se_plan* plan, int out[8]);
+
+/* Energy E = 1/2 sum phi q (FP64, on the device), into *d_energy (device double). */
+int se_energy(se_plan* plan, int N, const double* d_phi, const double* d_q, double* d_energy);
+
+const char* se_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,294 @@
+"""MI355X-native Spectral Ewald fast Gaussian gridding.
+
+Python mirror of the reference's C interface (davoudss/SE_fast_gridding:
+SE_fgg.h, SE_general.h, spectral_ewald.c) over the C ABI of
+``se_fast_gridding_amd/lib/libse_fgg_amd.so`` (hand-written gfx950 HIP kernels).
+
+* ``capi()``           -- the ctypes handle with argument types for every symbol in include/*.h
+* ``SE_FGG_params``, ``SE_FGG_work``, ``SE_state``, ``SE_opt`` -- byte layouts of the reference structs
+* ``Plan``             -- the device-resident engine (include/se_gpu.h) over torch CUDA tensors
+
+There is no CPU fallback: if the shared library is missing the import of ``capi()``
+raises, and every compute entry point runs on the GPU.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libse_fgg_amd.so")
+ROOT = os.path.dirname(HERE)
+INCLUDE_DIR = os.path.join(ROOT, "include")
+
+SE_OK, SE_ERR_ARG, SE_ERR_HIP, SE_ERR_FFT, SE_ERR_CAPACITY, SE_ERR_RANGE = 0, -1, -2, -3, -4, -5
+PREC_FP64, PREC_FP32 = 0, 1
+Q_POTENTIAL, Q_FORCE = 1, 2
+
+
+class SE_FGG_params(C.Structure):
+    """SE_fgg.h:121-134."""
+    _fields_ = [("N", C.c_int), ("P", C.c_int), ("P_half", C.c_int),
+                ("dims", C.c_int * 3), ("npdims", C.c_int * 3),
+                ("c", C.c_double), ("d", C.c_double), ("h", C.c_double),
+                ("a", C.c_double), ("b", C.c_double)]
+
+
+class SE_FGG_work(C.Structure):
+    """SE_fgg.h:100-118."""
+    _fields_ = [("H", C.POINTER(C.c_double)), ("zs", C.POINTER(C.c_double)),
+                ("zx", C.POINTER(C.c_double)), ("zy", C.POINTER(C.c_double)),
+                ("zz", C.POINTER(C.c_double)), ("idx", C.POINTER(C.c_int)),
+                ("zfx", C.POINTER(C.c_double)), ("zfy", C.POINTER(C.c_double)),
+                ("zfz", C.POINTER(C.c_double)),
+                ("free_zs", C.c_int), ("free_fgg_expa", C.c_int)]
+
+
+class SE_state(C.Structure):
+    """SE_fgg.h:142-148."""
+    _fields_ = [("x", C.POINTER(C.c_double)), ("q", C.POINTER(C.c_double)),
+                ("phi", C.POINTER(C.c_double))]
+
+
+class SE_opt(C.Structure):
+    """SE_general.h:20-24."""
+    _fields_ = [("m", C.c_double), ("c", C.c_double), ("box", C.c_double * 3),
+                ("h", C.c_double), ("xi", C.c_double), ("w", C.c_double),
+                ("eta", C.c_double), ("P", C.c_int), ("M", C.c_int), ("N", C.c_int)]
+
+
+class se_plan_desc(C.Structure):
+    """se_gpu.h se_plan_desc."""
+    _fields_ = [("periodicity", C.c_int), ("M", C.c_int), ("P", C.c_int),
+                ("box", C.c_double), ("xi", C.c_double), ("m", C.c_double),
+                ("precision", C.c_int), ("n_max", C.c_int), ("tile", C.c_int),
+                ("chunk", C.c_int), ("block", C.c_int)]
+
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_PP = C.POINTER(SE_FGG_params)
+_WP = C.POINTER(SE_FGG_work)
+_SP = C.POINTER(SE_state)
+_VP = C.c_void_p
+
+# name -> (restype, argtypes) for every exported symbol declared in include/*.h
+_SIG = {
+    # se_fgg.h
+    "se_fgg_set_periodicity": (None, [C.c_int]),
+    "se_fgg_get_periodicity": (C.c_int, []),
+    "SE_FGG_pack_params": (None, [_PP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]),
+    "SE2P_FGG_pack_params": (None, [_PP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_double]),
+    "SE1P_FGG_pack_params": (None, [_PP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_double]),
+    "SE_FGG_allocate_workspace": (None, [_WP, _PP, C.c_int, C.c_int]),
+    "SE_FGG_allocate_workspace_SSE_force": (None, [_WP, _PP, C.c_int, C.c_int]),
+    "SE_FGG_allocate_grid": (_D, [_PP]),
+    "SE_FGG_allocate_vec": (_D, [C.c_int]),
+    "SE_FGG_free_workspace": (None, [_WP]),
+    "SE_FGG_free_workspace_SSE_force": (None, [_WP]),
+    "SE_FGG_grid": (None, [_WP, _SP, _PP]),
+    "SE_FGG_expand_all": (None, [_WP, _SP, _PP]),
+    "SE_FGG_expand_all_SSE_force": (None, [_WP, _SP, _PP]),
+    "SE_FGG_int": (None, [_D, _WP, _SP, _PP]),
+    "SE_FGG_int_force": (None, [_D, _WP, _SP, _PP]),
+    "SE_FGG_base_gaussian": (None, [_WP, _PP]),
+    "SE_FGG_wrap_fcn": (None, [_D, _WP, _PP]),
+    "SE2P_FGG_wrap_fcn": (None, [_D, _WP, _PP]),
+    "SE1P_FGG_wrap_fcn": (None, [_D, _WP, _PP]),
+    "SE_FGG_extend_fcn": (None, [_WP, _D, _PP]),
+    "SE2P_FGG_extend_fcn": (None, [_WP, _D, _PP]),
+    "SE1P_FGG_extend_fcn": (None, [_WP, _D, _PP]),
+    "SE_init_system": (None, [_SP, _PP]),
+    "SE_init_unit_system": (None, [_SP, _PP]),
+    "SE_free_system": (None, [_SP]),
+    "SE_gettime": (C.c_double, []),
+    "SE_prod3": (C.c_int, [_I]),
+    "SE_fp_set_zero": (None, [_D, C.c_int]),
+    "SE_FGG_reorder_system": (None, [_SP, _WP, _PP]),
+    "calc_energy": (C.c_double, [SE_state, C.c_int]),
+    # se_general.h
+    "parse_params": (None, [C.POINTER(SE_opt), C.c_double]),
+    "k_vec": (None, [C.c_int, _D, _D, _D, _D]),
+    "scaling": (None, [C.c_double, _D, C.c_int, C.c_int, C.c_int, _D]),
+    "product_rc": (None, [_D, _D, _D, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "SE_FGG_FCN_params": (None, [_PP, C.POINTER(SE_opt), C.c_int]),
+    "SE_fgg_int_force": (None, [_D, _D, _D, C.c_int, SE_opt, _D]),
+    "copy_r2c": (None, [_D, _D, C.c_int]),
+    "copy_c2r": (None, [_D, _D, C.c_int]),
+    "SE_init_params": (C.c_double, [_I, _I, _D, C.c_double, C.c_double, C.c_double, _D, C.c_int]),
+    "spectral_ewald": (C.c_int, [_D, _D, SE_opt, C.c_double, _D, _D]),
+    "se_set_quantity": (None, [C.c_int]),
+    "se_set_precision": (None, [C.c_int]),
+    # se_gpu.h
+    "se_plan_create": (C.c_int, [C.POINTER(_VP), C.POINTER(se_plan_desc), _VP]),
+    "se_plan_destroy": (C.c_int, [_VP]),
+    "se_plan_set_stream": (C.c_int, [_VP, _VP]),
+    "se_plan_bytes": (C.c_size_t, [_VP]),
+    "se_plan_bytes_for": (C.c_size_t, [C.POINTER(se_plan_desc)]),
+    "se_solve": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP]),
+    "se_spread": (C.c_int, [_VP, C.c_int, _VP, _VP]),
+    "se_kspace": (C.c_int, [_VP]),
+    "se_interp": (C.c_int, [_VP, _VP, _VP]),
+    "se_plan_grid": (C.c_int, [_VP, C.POINTER(_VP), _I]),
+    "se_plan_enable_timing": (C.c_int, [_VP, C.c_int]),
+    "se_plan_timings": (C.c_int, [_VP, C.POINTER(C.c_float)]),
+    "se_plan_info": (C.c_int, [_VP, _I]),
+    "se_energy": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
+    "se_gpu_last_error": (C.c_char_p, []),
+}
+
+_SPREAD_SPLIT = ["SE_FGG_grid_split_SSE_dispatch", "SE_FGG_grid_split", "SE_FGG_grid_split_SSE",
+                 "SE_FGG_grid_split_SSE_u8", "SE_FGG_grid_split_SSE_P16", "SE_FGG_grid_split_AVX_dispatch",
+                 "SE_FGG_grid_split_AVX", "SE_FGG_grid_split_AVX_P16", "SE_FGG_grid_split_AVX_P8",
+                 "SE_FGG_grid_split_AVX_u8"]
+_SPREAD_SPLIT += [n.replace("SE_FGG_grid_split", "SE_FGG_grid_split") + "_force" for n in
+                  ["SE_FGG_grid_split_SSE_dispatch", "SE_FGG_grid_split", "SE_FGG_grid_split_SSE",
+                   "SE_FGG_grid_split_SSE_u8", "SE_FGG_grid_split_SSE_P16", "SE_FGG_grid_split_AVX_dispatch",
+                   "SE_FGG_grid_split_AVX_P16", "SE_FGG_grid_split_AVX_P8", "SE_FGG_grid_split_AVX_u8",
+                   "SE_FGG_grid_split_AVX"]]
+_INT_SPLIT = ["SE_FGG_int_split_SSE_dispatch", "SE_FGG_int_split", "SE_FGG_int_split_SSE",
+              "SE_FGG_int_split_SSE_u8", "SE_FGG_int_split_SSE_P8", "SE_FGG_int_split_SSE_P16",
+              "SE_FGG_int_split_AVX_dispatch", "SE_FGG_int_split_AVX", "SE_FGG_int_split_AVX_P8",
+              "SE_FGG_int_split_AVX_P16", "SE_FGG_int_split_AVX_u8"]
+_INT_SPLIT_FORCE = ["SE_FGG_int_split_SSE_dispatch_force", "SE_FGG_int_split_force",
+                    "SE_FGG_int_split_SSE_force", "SE_FGG_int_split_SSE_P8_force",
+                    "SE_FGG_int_split_SSE_u8_force", "SE_FGG_int_split_SSE_P16_force",
+                    "SE_FGG_int_split_AVX_dispatch_force", "SE_FGG_int_split_AVX_P16_force",
+                    "SE_FGG_int_split_AVX_P8_force", "SE_FGG_int_split_AVX_u8_force",
+                    "SE_FGG_int_split_AVX_force"]
+for _n in _SPREAD_SPLIT:
+    _SIG[_n] = (None, [_WP, _SP, _PP])
+for _n in _INT_SPLIT:
+    _SIG[_n] = (None, [_D, _WP, _PP])
+for _n in _INT_SPLIT_FORCE:
+    _SIG[_n] = (None, [_D, _SP, _WP, _PP])
+
+EXPORTS = sorted(_SIG)
+
+_capi = None
+
+
+def capi():
+    """Load libse_fgg_amd.so (built by ``__graft_entry__.build()``); raises if it is missing."""
+    global _capi
+    if _capi is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libse_fgg_amd.so is not built (%s): run __graft_entry__.build()" % LIB_PATH)
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIG.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _capi = lib
+    return _capi
+
+
+def last_error():
+    msg = capi().se_gpu_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc, what="se_gpu call"):
+    if rc != SE_OK:
+        raise RuntimeError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Plan:
+    """Device-resident SE3P solver (include/se_gpu.h) driven with torch CUDA tensors.
+
+    x is (3N,) float64 SoA [x..., y..., z...], q is (N,) float64, outputs are float64,
+    force SoA (3N,) in the reference's (q/2) grad(phi) convention (SE_fgg.c:2291-2307)."""
+
+    def __init__(self, M, P, box, xi, n_max, precision=PREC_FP64, m=0.0, tile=0, chunk=0, block=0,
+                 stream=None):
+        import torch
+        self._torch = torch
+        d = se_plan_desc(3, M, P, float(box), float(xi), float(m), precision, int(n_max), tile, chunk, block)
+        self.desc = d
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        h = C.c_void_p()
+        check(capi().se_plan_create(C.byref(h), C.byref(d), C.c_void_p(self.stream.cuda_stream)), "se_plan_create")
+        self.h = h
+        self.M, self.P, self.n_max, self.precision = M, P, n_max, precision
+
+    def close(self):
+        if getattr(self, "h", None):
+            capi().se_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def solve(self, x, q, phi=None, force=None):
+        N = q.numel()
+        check(capi().se_solve(self.h, N, _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_solve")
+
+    def spread(self, x, q):
+        check(capi().se_spread(self.h, q.numel(), _ptr(x), _ptr(q)), "se_spread")
+
+    def kspace(self):
+        check(capi().se_kspace(self.h), "se_kspace")
+
+    def interp(self, phi=None, force=None):
+        check(capi().se_interp(self.h, _ptr(phi), _ptr(force)), "se_interp")
+
+    def _grid_ptr(self):
+        p = C.c_void_p()
+        es = C.c_int()
+        check(capi().se_plan_grid(self.h, C.byref(p), C.byref(es)), "se_plan_grid")
+        dt = self._torch.float32 if es.value == 4 else self._torch.float64
+        return p.value, dt, es.value
+
+    def grid(self):
+        """Copy of the plan's periodic grid, (M, M, M) row-major (z fastest)."""
+        torch = self._torch
+        ptr, dt, es = self._grid_ptr()
+        out = torch.empty((self.M, self.M, self.M), dtype=dt, device="cuda")
+        torch.cuda.synchronize()
+        _hip_copy(out.data_ptr(), ptr, out.numel() * es)
+        return out
+
+    def set_grid(self, g):
+        """Overwrite the plan's periodic grid (e.g. to interpolate a given field)."""
+        torch = self._torch
+        ptr, dt, es = self._grid_ptr()
+        src = g.to(dtype=dt).contiguous()
+        assert src.numel() == self.M ** 3
+        torch.cuda.synchronize()
+        _hip_copy(ptr, src.data_ptr(), src.numel() * es)
+
+    def enable_timing(self, on=True):
+        check(capi().se_plan_enable_timing(self.h, int(on)), "se_plan_enable_timing")
+
+    def timings(self):
+        ms = (C.c_float * 8)()
+        check(capi().se_plan_timings(self.h, ms), "se_plan_timings")
+        keys = ["spread_stage", "spread_kernel", "merge", "kspace", "interp_stage", "gather_kernel", "bucketing"]
+        return {k: float(ms[i]) for i, k in enumerate(keys)}
+
+    def info(self):
+        out = (C.c_int * 8)()
+        check(capi().se_plan_info(self.h, out), "se_plan_info")
+        keys = ["tile", "chunk", "block", "ntiles", "items", "discarded", "S", "max_items"]
+        return {k: int(out[i]) for i, k in enumerate(keys)}
+
+    def nbytes(self):
+        return int(capi().se_plan_bytes(self.h))
+
+
+_hip = None
+
+
+def _hip_copy(dst, src, nbytes):
+    """Synchronous device-to-device copy through the HIP runtime the library links."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so", mode=C.RTLD_GLOBAL)
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    rc = _hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, 3)   # hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed (%d)" % rc)

@@ -1,0 +1,497 @@
+// Device-resident SE3P engine (se_gpu.h): plan = grid geometry + tiling + HBM
+// scratch + rocFFT plans; stages spread / kspace / interp on one HIP stream.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/se_gpu.h"
+#include "se_internal.h"
+
+namespace se {
+
+static thread_local std::string g_last_error;
+static std::mutex g_err_mu;
+static std::string g_last_error_global;
+
+void set_error(const std::string& msg)
+{
+    g_last_error = msg;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_last_error_global = msg;
+}
+
+const char* last_error()
+{
+    if (!g_last_error.empty()) return g_last_error.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_last_error_global.c_str();
+}
+
+static void rocfft_init_once()
+{
+    static std::once_flag f;
+    std::call_once(f, [] { rocfft_setup(); });
+}
+
+#define SE_FFT(expr)                                                                             \
+    do {                                                                                         \
+        rocfft_status s_ = (expr);                                                               \
+        if (s_ != rocfft_status_success)                                                         \
+            throw ::se::Error{SE_ERR_FFT, std::string(#expr) + " failed (" + std::to_string((int)s_) + ")"}; \
+    } while (0)
+
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+// Tiling rule: largest tile side (<= 16) whose S^3 LDS sub-grid fits the budget;
+// FP32 targets two work-groups per CU, FP64 one.
+static Tiling choose_tiling(int esize, int P, int M, int n_cap, int tile, int chunk, int block, int lds_max)
+{
+    Tiling tl{};
+    const int budget = std::min(lds_max, esize == 4 ? 64 * 1024 : 160 * 1024);
+    int TS = tile;
+    if (TS <= 0) {
+        TS = 0;
+        for (int t = 16; t >= 1; --t) {
+            long S = t + P - 1;
+            if (S * S * S * esize <= budget) { TS = t; break; }
+        }
+        if (TS == 0) {
+            for (int t = 16; t >= 1; --t) {
+                long S = t + P - 1;
+                if (S * S * S * esize <= lds_max) { TS = t; break; }
+            }
+        }
+        SE_REQUIRE(TS > 0, SE_ERR_ARG, "support P too large for the LDS sub-grid at this precision");
+    }
+    TS = std::min(TS, M);
+    SE_REQUIRE(TS >= 1 && TS <= 255, SE_ERR_ARG, "tile side out of range");
+    tl.TS = TS;
+    tl.NT = (M + TS - 1) / TS;
+    tl.S = TS + P - 1;
+    tl.Szp = tl.S;
+    SE_REQUIRE((long)tl.S * tl.S * tl.Szp * esize <= lds_max, SE_ERR_ARG, "tile sub-grid exceeds LDS");
+    tl.ntiles = tl.NT * tl.NT * tl.NT;
+    tl.chunk = chunk > 0 ? chunk : 2048;
+    tl.block = block > 0 ? block : 512;
+    SE_REQUIRE(tl.block % 64 == 0 && tl.block <= 512, SE_ERR_ARG, "block must be a multiple of 64, <= 512");
+    tl.sub_elems = tl.S * tl.S * tl.Szp;
+    tl.part_elems = tl.S * tl.S * tl.S;
+    tl.max_items = tl.ntiles + n_cap / tl.chunk + 1;
+    return tl;
+}
+
+}  // namespace se
+
+using namespace se;
+
+struct se_plan {
+    se_plan_desc desc{};
+    int esize = 8;
+    Geom g{};
+    Tiling tl{};
+    double L = 1, xi = 1, m = 1, eta = 0;
+    hipStream_t stream = nullptr;
+    int n_cap = 0;
+    int last_N = -1;
+    // device buffers
+    int *key = nullptr, *rank = nullptr, *count = nullptr, *tile_start = nullptr, *nchunk = nullptr,
+        *chunk_start = nullptr, *item_tile = nullptr, *item_beg = nullptr, *item_end = nullptr,
+        *n_items = nullptr, *ncover = nullptr;
+    int2* cover = nullptr;
+    void *rec = nullptr, *partials = nullptr, *grid = nullptr, *spec = nullptr, *cub_tmp = nullptr,
+         *fft_work = nullptr;
+    double* gtab = nullptr;
+    size_t cub_bytes = 0, fft_work_bytes = 0, bytes = 0;
+    rocfft_plan fwd = nullptr, bwd = nullptr;
+    rocfft_execution_info info = nullptr;
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    std::vector<void*> allocs;
+
+    void* alloc(size_t b)
+    {
+        void* p = nullptr;
+        if (b == 0) b = 16;
+        SE_HIP(hipMalloc(&p, b));
+        allocs.push_back(p);
+        bytes += b;
+        return p;
+    }
+    ~se_plan()
+    {
+        if (fwd) rocfft_plan_destroy(fwd);
+        if (bwd) rocfft_plan_destroy(bwd);
+        if (info) rocfft_execution_info_destroy(info);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (void* p : allocs) (void)hipFree(p);
+    }
+    void rec_ev(int i)
+    {
+        if (timing) SE_HIP(hipEventRecord(ev[i], stream));
+    }
+};
+
+namespace {
+
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        return SE_OK;
+    } catch (const se::Error& e) {
+        se::set_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        se::set_error(e.what());
+        return SE_ERR_ARG;
+    }
+}
+
+void derive(const se_plan_desc* d, Geom& g, double& eta, double& m)
+{
+    SE_REQUIRE(d->M >= 1 && d->P >= 1 && d->P <= 32 && d->box > 0 && d->xi > 0, SE_ERR_ARG,
+               "invalid plan description (need M>=1, 1<=P<=32, box>0, xi>0)");
+    m = d->m > 0 ? d->m : 0.95 * std::sqrt(M_PI * d->P);
+    // parse_params (SE_general.c:16-29)
+    const double h = d->box / (double)d->M;
+    const double w = (double)d->P * h / 2.0;
+    eta = (2.0 * w * d->xi / m) * (2.0 * w * d->xi / m);
+    const double c = 2.0 * d->xi * d->xi / eta;
+    g.M = d->M;
+    g.P = d->P;
+    g.odd = d->P & 1;
+    const int ph = g.odd ? (d->P - 1) / 2 : d->P / 2;
+    g.first = g.odd ? -ph : -ph + 1;
+    g.h = h;
+    g.c = c;
+    g.d = std::pow(c / M_PI, 1.5);
+}
+
+void build_cover(se_plan* pl)
+{
+    const int M = pl->g.M, TS = pl->tl.TS, NT = pl->tl.NT, S = pl->tl.S;
+    std::vector<std::vector<int2>> lists(M);
+    int ncov = 0;
+    for (int g = 0; g < M; ++g) {
+        for (int t = 0; t < NT; ++t) {
+            int l = ((g - t * TS) % M + M) % M;
+            for (; l < S; l += M) lists[g].push_back(make_int2(t, l));
+        }
+        ncov = std::max<int>(ncov, (int)lists[g].size());
+    }
+    pl->tl.ncov = std::max(ncov, 1);
+    std::vector<int2> flat((size_t)M * pl->tl.ncov, make_int2(0, 0));
+    std::vector<int> cnt(M);
+    for (int g = 0; g < M; ++g) {
+        cnt[g] = (int)lists[g].size();
+        for (size_t a = 0; a < lists[g].size(); ++a) flat[(size_t)g * pl->tl.ncov + a] = lists[g][a];
+    }
+    pl->cover = (int2*)pl->alloc(flat.size() * sizeof(int2));
+    pl->ncover = (int*)pl->alloc(M * sizeof(int));
+    SE_HIP(hipMemcpy(pl->cover, flat.data(), flat.size() * sizeof(int2), hipMemcpyHostToDevice));
+    SE_HIP(hipMemcpy(pl->ncover, cnt.data(), M * sizeof(int), hipMemcpyHostToDevice));
+}
+
+size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max)
+{
+    Geom g;
+    double eta, m;
+    derive(d, g, eta, m);
+    const int es = d->precision == SE_PREC_FP32 ? 4 : 8;
+    Tiling tl = choose_tiling(es, d->P, d->M, d->n_max, d->tile, d->chunk, d->block, lds_max);
+    const size_t M = d->M, N = d->n_max;
+    size_t b = 0;
+    b += 2 * N * 4;                                   // key, rank
+    b += 5 * (size_t)(tl.ntiles + 1) * 4;             // count, tile_start, nchunk, chunk_start
+    b += 3 * (size_t)tl.max_items * 4;                // items
+    b += N * (es == 4 ? 32 : 48);                     // records
+    b += (size_t)tl.max_items * tl.part_elems * es;   // partials
+    b += M * M * M * es;                              // grid
+    b += M * M * (M / 2 + 1) * 2 * es;                // half spectrum
+    b += M * M * (M / 2 + 1) * 2 * es;                // FFT work (upper estimate)
+    return b;
+}
+
+int lds_limit()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+        return 64 * 1024;
+    return v;
+}
+
+template <typename T>
+void run_spread(se_plan* pl, int N, const double* x, const double* q)
+{
+    hipStream_t s = pl->stream;
+    const Tiling& tl = pl->tl;
+    pl->rec_ev(0);
+    launch_fill_i32(pl->count, tl.ntiles + 1, 0, s);
+    launch_bin_keys<T>(x, N, pl->g, tl, pl->key, pl->rank, pl->count, s);
+    launch_nchunk(pl->count, tl.ntiles, tl.chunk, pl->nchunk, s);
+    size_t tb = pl->cub_bytes;
+    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->count, pl->tile_start, tl.ntiles + 1, s));
+    tb = pl->cub_bytes;
+    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->nchunk, pl->chunk_start, tl.ntiles + 1, s));
+    launch_items(pl->count, pl->tile_start, pl->chunk_start, tl.ntiles, tl.chunk, pl->item_tile, pl->item_beg,
+                 pl->item_end, pl->n_items, s);
+    launch_scatter<T>(x, q, N, pl->g, tl, pl->key, pl->rank, pl->tile_start, (Rec<T>*)pl->rec, s);
+    pl->rec_ev(1);
+    launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items, pl->g,
+                           tl, (T*)pl->partials, s);
+    pl->rec_ev(2);
+    launch_merge<T>((const T*)pl->partials, pl->chunk_start, pl->cover, pl->ncover, pl->g, tl, (T*)pl->grid, s);
+    pl->rec_ev(3);
+    SE_HIP(hipGetLastError());
+}
+
+template <typename T>
+void run_interp(se_plan* pl, double* phi, double* force)
+{
+    pl->rec_ev(4);
+    launch_interp_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
+                           (const T*)pl->grid, pl->g, pl->tl, pl->last_N, phi, force, pl->stream);
+    pl->rec_ev(5);
+    SE_HIP(hipGetLastError());
+}
+
+void run_kspace(se_plan* pl)
+{
+    pl->rec_ev(6);
+    void* in[1] = {pl->grid};
+    void* out[1] = {pl->spec};
+    SE_FFT(rocfft_execute(pl->fwd, in, out, pl->info));
+    if (pl->esize == 4)
+        launch_green<float>(pl->spec, pl->g.M, pl->gtab, pl->stream);
+    else
+        launch_green<double>(pl->spec, pl->g.M, pl->gtab, pl->stream);
+    void* in2[1] = {pl->spec};
+    void* out2[1] = {pl->grid};
+    SE_FFT(rocfft_execute(pl->bwd, in2, out2, pl->info));
+    pl->rec_ev(7);
+    SE_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* se_gpu_last_error(void) { return se::last_error(); }
+
+size_t se_plan_bytes_for(const se_plan_desc* desc)
+{
+    size_t b = 0;
+    int rc = guarded([&] { b = plan_bytes_estimate(desc, lds_limit()); });
+    return rc == SE_OK ? b : 0;
+}
+
+int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
+{
+    if (!out || !d) return SE_ERR_ARG;
+    *out = nullptr;
+    se_plan* pl = new se_plan();
+    int rc = guarded([&] {
+        SE_REQUIRE(d->periodicity == 3, SE_ERR_ARG, "se_plan: only periodicity 3 (SE3P) has a fused solver");
+        SE_REQUIRE(d->n_max >= 0, SE_ERR_ARG, "n_max must be >= 0");
+        pl->desc = *d;
+        pl->esize = d->precision == SE_PREC_FP32 ? 4 : 8;
+        derive(d, pl->g, pl->eta, pl->m);
+        pl->L = d->box;
+        pl->xi = d->xi;
+        pl->stream = (hipStream_t)stream;
+        pl->n_cap = std::max(d->n_max, 1);
+        const int lds_max = lds_limit();
+        pl->tl = choose_tiling(pl->esize, d->P, d->M, pl->n_cap, d->tile, d->chunk, d->block, lds_max);
+        const Tiling& tl = pl->tl;
+        const size_t M = d->M, N = pl->n_cap;
+        pl->key = (int*)pl->alloc(N * 4);
+        pl->rank = (int*)pl->alloc(N * 4);
+        pl->count = (int*)pl->alloc((tl.ntiles + 1) * 4);
+        pl->tile_start = (int*)pl->alloc((tl.ntiles + 1) * 4);
+        pl->nchunk = (int*)pl->alloc((tl.ntiles + 1) * 4);
+        pl->chunk_start = (int*)pl->alloc((tl.ntiles + 2) * 4);
+        pl->item_tile = (int*)pl->alloc((size_t)tl.max_items * 4);
+        pl->item_beg = (int*)pl->alloc((size_t)tl.max_items * 4);
+        pl->item_end = (int*)pl->alloc((size_t)tl.max_items * 4);
+        pl->n_items = (int*)pl->alloc(16);
+        pl->rec = pl->alloc(N * (pl->esize == 4 ? sizeof(Rec<float>) : sizeof(Rec<double>)));
+        pl->partials = pl->alloc((size_t)tl.max_items * tl.part_elems * pl->esize);
+        pl->grid = pl->alloc(M * M * M * pl->esize);
+        pl->spec = pl->alloc(M * M * (M / 2 + 1) * 2 * pl->esize);
+        pl->gtab = (double*)pl->alloc((2 * M + 1) * sizeof(double));
+        SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->count, pl->tile_start, tl.ntiles + 1,
+                                                pl->stream));
+        pl->cub_tmp = pl->alloc(pl->cub_bytes);
+        build_cover(pl);
+        // Green's tables: s = -(1-eta)/(4 xi^2)   (spectral_ewald.c:126)
+        launch_green_tables(pl->gtab, d->M, d->box, -(1.0 - pl->eta) / (4.0 * d->xi * d->xi), pl->stream);
+        // rocFFT R2C / C2R, lengths fastest-first (z, y, x)
+        rocfft_init_once();
+        size_t lens[3] = {M, M, M};
+        rocfft_precision prec = pl->esize == 4 ? rocfft_precision_single : rocfft_precision_double;
+        SE_FFT(rocfft_plan_create(&pl->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, prec, 3,
+                                  lens, 1, nullptr));
+        SE_FFT(rocfft_plan_create(&pl->bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, prec, 3,
+                                  lens, 1, nullptr));
+        size_t w1 = 0, w2 = 0;
+        SE_FFT(rocfft_plan_get_work_buffer_size(pl->fwd, &w1));
+        SE_FFT(rocfft_plan_get_work_buffer_size(pl->bwd, &w2));
+        pl->fft_work_bytes = std::max(w1, w2);
+        SE_FFT(rocfft_execution_info_create(&pl->info));
+        if (pl->fft_work_bytes) {
+            pl->fft_work = pl->alloc(pl->fft_work_bytes);
+            SE_FFT(rocfft_execution_info_set_work_buffer(pl->info, pl->fft_work, pl->fft_work_bytes));
+        }
+        SE_FFT(rocfft_execution_info_set_stream(pl->info, pl->stream));
+        for (auto& e : pl->ev) SE_HIP(hipEventCreate(&e));
+        SE_HIP(hipStreamSynchronize(pl->stream));
+    });
+    if (rc != SE_OK) {
+        delete pl;
+        return rc;
+    }
+    *out = pl;
+    return SE_OK;
+}
+
+int se_plan_destroy(se_plan* pl)
+{
+    if (!pl) return SE_OK;
+    int rc = guarded([&] { SE_HIP(hipStreamSynchronize(pl->stream)); });
+    delete pl;
+    return rc;
+}
+
+int se_plan_set_stream(se_plan* pl, void* stream)
+{
+    if (!pl) return SE_ERR_ARG;
+    return guarded([&] {
+        pl->stream = (hipStream_t)stream;
+        SE_FFT(rocfft_execution_info_set_stream(pl->info, pl->stream));
+    });
+}
+
+size_t se_plan_bytes(const se_plan* pl) { return pl ? pl->bytes : 0; }
+
+int se_spread(se_plan* pl, int N, const double* x, const double* q)
+{
+    if (!pl) return SE_ERR_ARG;
+    return guarded([&] {
+        SE_REQUIRE(N >= 0 && N <= pl->n_cap, SE_ERR_CAPACITY, "N exceeds the plan's particle capacity");
+        SE_REQUIRE(N == 0 || (x && q), SE_ERR_ARG, "null particle arrays");
+        pl->last_N = N;
+        if (pl->esize == 4)
+            run_spread<float>(pl, N, x, q);
+        else
+            run_spread<double>(pl, N, x, q);
+    });
+}
+
+int se_kspace(se_plan* pl)
+{
+    if (!pl) return SE_ERR_ARG;
+    return guarded([&] { run_kspace(pl); });
+}
+
+int se_interp(se_plan* pl, double* phi, double* force)
+{
+    if (!pl) return SE_ERR_ARG;
+    return guarded([&] {
+        SE_REQUIRE(pl->last_N >= 0, SE_ERR_ARG, "se_interp before se_spread");
+        if (!phi && !force) return;
+        if (pl->esize == 4)
+            run_interp<float>(pl, phi, force);
+        else
+            run_interp<double>(pl, phi, force);
+    });
+}
+
+int se_solve(se_plan* pl, int N, const double* x, const double* q, double* phi, double* force)
+{
+    int rc = se_spread(pl, N, x, q);
+    if (rc) return rc;
+    rc = se_kspace(pl);
+    if (rc) return rc;
+    return se_interp(pl, phi, force);
+}
+
+int se_plan_grid(se_plan* pl, void** d_grid, int* elem_bytes)
+{
+    if (!pl || !d_grid) return SE_ERR_ARG;
+    *d_grid = pl->grid;
+    if (elem_bytes) *elem_bytes = pl->esize;
+    return SE_OK;
+}
+
+int se_plan_enable_timing(se_plan* pl, int on)
+{
+    if (!pl) return SE_ERR_ARG;
+    pl->timing = on != 0;
+    return SE_OK;
+}
+
+int se_plan_timings(se_plan* pl, float ms[8])
+{
+    if (!pl || !ms) return SE_ERR_ARG;
+    return guarded([&] {
+        SE_HIP(hipStreamSynchronize(pl->stream));
+        for (int i = 0; i < 8; ++i) ms[i] = 0.f;
+        auto el = [&](int a, int b) {
+            float v = 0.f;
+            if (hipEventElapsedTime(&v, pl->ev[a], pl->ev[b]) != hipSuccess) v = -1.f;
+            return v;
+        };
+        ms[0] = el(0, 3);
+        ms[1] = el(1, 2);
+        ms[2] = el(2, 3);
+        ms[3] = el(6, 7);
+        ms[4] = el(4, 5);
+        ms[5] = ms[4];
+        ms[6] = el(0, 1);
+        (void)hipGetLastError();
+    });
+}
+
+int se_plan_info(se_plan* pl, int out[8])
+{
+    if (!pl || !out) return SE_ERR_ARG;
+    return guarded([&] {
+        for (int i = 0; i < 8; ++i) out[i] = 0;
+        out[0] = pl->tl.TS;
+        out[1] = pl->tl.chunk;
+        out[2] = pl->tl.block;
+        out[3] = pl->tl.ntiles;
+        SE_HIP(hipStreamSynchronize(pl->stream));
+        int ni = 0, bad = 0;
+        SE_HIP(hipMemcpy(&ni, pl->n_items, sizeof(int), hipMemcpyDeviceToHost));
+        SE_HIP(hipMemcpy(&bad, pl->count + pl->tl.ntiles, sizeof(int), hipMemcpyDeviceToHost));
+        out[4] = ni;
+        out[5] = bad;
+        out[6] = pl->tl.S;
+        out[7] = pl->tl.max_items;
+    });
+}
+
+int se_energy(se_plan* pl, int N, const double* phi, const double* q, double* d_energy)
+{
+    if (!pl) return SE_ERR_ARG;
+    return guarded([&] {
+        launch_energy(phi, q, N, d_energy, pl->stream);
+        SE_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

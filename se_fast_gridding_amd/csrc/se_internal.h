@@ -1,0 +1,133 @@
+// Internal declarations shared by the HIP translation units of libse_fgg_amd.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace se {
+
+// ---------------------------------------------------------------------------------
+// Errors: every HIP call goes through SE_HIP; failures are recorded for
+// se_gpu_last_error() and turned into return codes by the C entry points.
+// ---------------------------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* last_error();
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define SE_HIP(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::se::Error{-2, std::string(#expr) + ": " + hipGetErrorString(e_)};   \
+    } while (0)
+
+#define SE_REQUIRE(cond, code, msg)                                                       \
+    do {                                                                                  \
+        if (!(cond)) throw ::se::Error{(code), std::string(msg)};                         \
+    } while (0)
+
+// ---------------------------------------------------------------------------------
+// Geometry of one cubic SE3P grid as the fused kernels see it.
+// Support point i of a particle sits at periodic index (iota + first + i) mod M,
+// iota = round(x/h) for odd P, floor(x/h) for even P (SE_fgg.c:554-573).
+// ---------------------------------------------------------------------------------
+struct Geom {
+    int M;        // grid points per side
+    int P;        // support
+    int first;    // -P_half (odd P) or 1-P_half (even P)
+    int odd;
+    double h, c, d;   // spacing, Gaussian exponent, normalisation (c/pi)^1.5
+};
+
+// Tile decomposition used by the bucketed spread/gather.  A tile of TS^3 window
+// start points owns an LDS sub-grid of S^3 points, S = TS + P - 1.
+struct Tiling {
+    int TS, NT, S;    // tile side, tiles per side (ceil(M/TS)), sub-grid side
+    int Szp;          // z pitch of the LDS sub-grid (>= S)
+    int ntiles;       // NT^3
+    int chunk;        // max particles per work item
+    int block;        // threads per work-group
+    int sub_elems;    // S*S*Szp
+    int part_elems;   // S*S*S
+    int max_items;    // ntiles + n_cap / chunk + 1
+    int ncov;         // max covering (tile, offset) pairs per coordinate
+};
+
+// Particle record written by the bucketing scatter, read by spread and gather.
+template <typename T>
+struct alignas(16) Rec {
+    T t[3];   // offset from the grid point iota*h (computed in FP64)
+    T q;      // charge
+    int lo;   // window start relative to the tile origin: x | y << 8 | z << 16
+    int orig; // index in the caller's arrays
+};
+
+// Parameters of the reference-layout (legacy) kernels: SE_FGG_params + periodicity.
+struct LParams {
+    int N, P, P_half, dims[3], npdims[3];
+    double c, d, h, a, b;
+    int per;
+};
+
+// ---- launchers (se_kernels.hip) ------------------------------------------------
+void launch_fill_i32(int* p, long n, int v, hipStream_t s);
+void launch_fill_bytes0(void* p, size_t bytes, hipStream_t s);
+
+template <typename T>
+void launch_bin_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key,
+                     int* rank, int* count, hipStream_t s);
+void launch_nchunk(const int* count, int ntiles, int chunk, int* nchunk, hipStream_t s);
+void launch_items(const int* count, const int* tile_start, const int* chunk_start, int ntiles,
+                  int chunk, int* item_tile, int* item_beg, int* item_end, int* n_items,
+                  hipStream_t s);
+template <typename T>
+void launch_scatter(const double* x, const double* q, int N, const Geom& g, const Tiling& tl,
+                    const int* key, const int* rank, const int* tile_start, Rec<T>* rec,
+                    hipStream_t s);
+template <typename T>
+void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
+                         const int* item_end, const int* n_items, const Geom& g,
+                         const Tiling& tl, T* partials, hipStream_t s);
+template <typename T>
+void launch_merge(const T* partials, const int* chunk_start, const int2* cover,
+                  const int* ncover, const Geom& g, const Tiling& tl, T* grid, hipStream_t s);
+template <typename T>
+void launch_interp_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
+                         const int* item_end, const int* n_items, const T* grid, const Geom& g,
+                         const Tiling& tl, int N, double* phi, double* force, hipStream_t s);
+template <typename T>
+void launch_green(void* spec, int M, const double* tab, hipStream_t s);
+void launch_green_tables(double* tab, int M, double L, double scalar, hipStream_t s);
+void launch_energy(const double* phi, const double* q, int N, double* out, hipStream_t s);
+
+// legacy (FP64, reference layout)
+void launch_base_gaussian(const LParams& p, double* zs, hipStream_t s);
+void launch_expand_all(const LParams& p, int force, const double* x, double* zx, double* zy,
+                       double* zz, int* idx, double* zfx, double* zfy, double* zfz,
+                       hipStream_t s);
+void launch_grid_split_ext(const LParams& p, const double* q, const double* zx,
+                           const double* zy, const double* zz, const int* idx,
+                           const double* zs, double* H, hipStream_t s);
+void launch_grid_ext(const LParams& p, const double* x, const double* q, const double* zs,
+                     double* H, hipStream_t s);
+void launch_int_split_ext(const LParams& p, int force, const double* H, const double* q,
+                          const double* zx, const double* zy, const double* zz,
+                          const double* zfx, const double* zfy, const double* zfz,
+                          const int* idx, const double* zs, double* out, hipStream_t s);
+void launch_int_ext(const LParams& p, int force, const double* H, const double* x,
+                    const double* q, const double* zs, double* out, hipStream_t s);
+void launch_wrap(const LParams& p, const double* H, double* Hper, hipStream_t s);
+void launch_extend(const LParams& p, const double* Hper, double* H, hipStream_t s);
+void launch_scaling(double scalar, double* Z, int n, double L, hipStream_t s);
+void launch_product_rc(const double* a, const double* b, double* c, long n, int flag,
+                       hipStream_t s);
+void launch_reorder_gather(const double* xin, const double* qin, const int* perm, int N,
+                           double* xout, double* qout, hipStream_t s);
+
+}  // namespace se

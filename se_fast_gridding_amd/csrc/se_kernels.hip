@@ -1,0 +1,1000 @@
+// Hand-written gfx950 kernels of libse_fgg_amd.so.
+//
+// Two families:
+//  * the fused engine (FP32 or FP64): bucketing by grid tile, tiled spread into
+//    LDS sub-grids with per-work-item partials, fixed-order halo merge into the
+//    periodic grid (wrap fused), Green's-function scaling on the R2C half
+//    spectrum, tiled gather (extend fused) with a wavefront reduction;
+//  * the reference-layout FP64 kernels behind the drop-in SE_FGG_* surface
+//    (expansion arrays, extended row-major grid, column-major periodic grid).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "se_internal.h"
+
+namespace se {
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+__device__ __forceinline__ int imod(int i, int n)
+{
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// Wave-uniform broadcast of lane l's value (v_readlane into an SGPR).
+__device__ __forceinline__ float bcast(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bcast(double v, int l)
+{
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// LDS accumulate without return value (ds_add_f32 / ds_add_f64).
+template <typename T>
+__device__ __forceinline__ void lds_add(T* p, T v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float dexp(float a) { return expf(a); }
+__device__ __forceinline__ double dexp(double a) { return exp(a); }
+
+// Kernels whose dynamic LDS exceeds the default per-launch limit must opt in once.
+static void allow_lds(const void* fn, size_t bytes)
+{
+    if (bytes <= 64 * 1024) return;
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// ================================================================================
+// Utility
+// ================================================================================
+__global__ void k_fill_i32(int* p, long n, int v)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void launch_fill_i32(int* p, long n, int v, hipStream_t s)
+{
+    if (n <= 0) return;
+    int blocks = (int)std::min<long>(cdiv(n, 256), 4096);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+
+__global__ void k_fill_u32(unsigned int* p, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+void launch_fill_bytes0(void* p, size_t bytes, hipStream_t s)
+{
+    long n = (long)(bytes / 4);
+    if (n <= 0) return;
+    int blocks = (int)std::min<long>(cdiv(n, 256), 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, (unsigned int*)p, n);
+}
+
+// ================================================================================
+// Bucketing: particle -> tile of its window start (counting sort)
+// ================================================================================
+__device__ __forceinline__ int window_start(double x, const Geom& g, double& t, bool& ok)
+{
+    double u = x / g.h;
+    ok = fabs(u) < 1.0e9;
+    if (!ok) u = 0.0;
+    int iota = g.odd ? (int)round(u) : (int)floor(u);
+    t = x - g.h * iota;
+    return imod(iota + g.first, g.M);
+}
+
+template <typename T>
+__global__ void k_bin_keys(const double* __restrict__ x, int N, Geom g, Tiling tl,
+                           int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    bool ok0, ok1, ok2;
+    double t;
+    int w0 = window_start(x[n], g, t, ok0);
+    int w1 = window_start(x[n + (long)N], g, t, ok1);
+    int w2 = window_start(x[n + 2l * N], g, t, ok2);
+    int k = (ok0 && ok1 && ok2)
+                ? ((w0 / tl.TS) * tl.NT + (w1 / tl.TS)) * tl.NT + (w2 / tl.TS)
+                : tl.ntiles;   // non-finite or absurd positions go to a discard bucket
+    key[n] = k;
+    rank[n] = atomicAdd(&count[k], 1);
+}
+
+template <typename T>
+void launch_bin_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key, int* rank,
+                     int* count, hipStream_t s)
+{
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_bin_keys<T>, dim3(cdiv(N, 256)), dim3(256), 0, s, x, N, g, tl, key, rank, count);
+}
+
+__global__ void k_nchunk(const int* __restrict__ count, int ntiles, int chunk, int* __restrict__ nchunk)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    nchunk[t] = (t < ntiles) ? (count[t] + chunk - 1) / chunk : 0;
+}
+
+void launch_nchunk(const int* count, int ntiles, int chunk, int* nchunk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nchunk, dim3(cdiv(ntiles + 1, 256)), dim3(256), 0, s, count, ntiles, chunk, nchunk);
+}
+
+__global__ void k_items(const int* __restrict__ count, const int* __restrict__ tile_start,
+                        const int* __restrict__ chunk_start, int ntiles, int chunk,
+                        int* __restrict__ item_tile, int* __restrict__ item_beg,
+                        int* __restrict__ item_end, int* __restrict__ n_items)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *n_items = chunk_start[ntiles];
+    if (t >= ntiles) return;
+    const int b = tile_start[t], e = b + count[t];
+    int it = chunk_start[t];
+    for (int p = b; p < e; p += chunk, ++it) {
+        item_tile[it] = t;
+        item_beg[it] = p;
+        item_end[it] = min(p + chunk, e);
+    }
+}
+
+void launch_items(const int* count, const int* tile_start, const int* chunk_start, int ntiles, int chunk,
+                  int* item_tile, int* item_beg, int* item_end, int* n_items, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_items, dim3(cdiv(ntiles, 256)), dim3(256), 0, s, count, tile_start, chunk_start,
+                       ntiles, chunk, item_tile, item_beg, item_end, n_items);
+}
+
+template <typename T>
+__global__ void k_scatter(const double* __restrict__ x, const double* __restrict__ q, int N, Geom g,
+                          Tiling tl, const int* __restrict__ key, const int* __restrict__ rank,
+                          const int* __restrict__ tile_start, Rec<T>* __restrict__ rec)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    bool ok;
+    double t0, t1, t2;
+    int w0 = window_start(x[n], g, t0, ok);
+    int w1 = window_start(x[n + (long)N], g, t1, ok);
+    int w2 = window_start(x[n + 2l * N], g, t2, ok);
+    const int k = key[n];
+    Rec<T> r;
+    r.t[0] = (T)t0;
+    r.t[1] = (T)t1;
+    r.t[2] = (T)t2;
+    r.q = (T)q[n];
+    r.lo = (w0 % tl.TS) | ((w1 % tl.TS) << 8) | ((w2 % tl.TS) << 16);
+    r.orig = n;
+    rec[tile_start[k] + rank[n]] = r;
+}
+
+template <typename T>
+void launch_scatter(const double* x, const double* q, int N, const Geom& g, const Tiling& tl, const int* key,
+                    const int* rank, const int* tile_start, Rec<T>* rec, hipStream_t s)
+{
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_scatter<T>, dim3(cdiv(N, 256)), dim3(256), 0, s, x, q, N, g, tl, key, rank,
+                       tile_start, rec);
+}
+
+// ================================================================================
+// Tiled spread.  One work item = up to `chunk` particles of one tile.  The
+// work-group zeroes an S^3 LDS sub-grid, each wave takes one particle at a time:
+// lanes 0..P-1 build the three 1-D Gaussian factors (q*d folded into x), lane l
+// owns (j,k) pairs l, l+64, ... of the P^2 window columns and adds
+// X[i]*Y[j]*Z[k] with ds_add for i = 0..P-1.  The sub-grid is then written to
+// the item's partial slot; the halo merge sums slots in fixed order.
+// ================================================================================
+// Lane l of a wave owns the window columns (j,k) = divmod(l + 64u, P), u < U,
+// U = ceil(P^2/64).  PT > 0 fixes P at compile time; PT == 0 takes P from the
+// geometry with ceil(P^2/64) <= UM.
+struct LaneCols {
+    int P, PP, lane;
+    __device__ LaneCols(int P_, int lane_) : P(P_), PP(P_ * P_), lane(lane_) {}
+    __device__ bool ok(int u) const { return lane + 64 * u < PP; }
+    __device__ int j(int u) const { return ok(u) ? (lane + 64 * u) / P : 0; }
+    __device__ int k(int u) const { return ok(u) ? (lane + 64 * u) - j(u) * P : 0; }
+};
+
+template <typename T, int PT, int UM>
+__global__ __launch_bounds__(512) void k_spread_tiles(
+    const Rec<T>* __restrict__ rec, const int* __restrict__ item_tile, const int* __restrict__ item_beg,
+    const int* __restrict__ item_end, const int* __restrict__ n_items, Geom g, Tiling tl,
+    T* __restrict__ partials)
+{
+    const int item = blockIdx.x;
+    if (item >= *n_items) return;   // uniform across the work-group
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* sub = reinterpret_cast<T*>(smem_raw);
+    const int P = PT > 0 ? PT : g.P;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int e = tid; e < tl.sub_elems; e += nthr) sub[e] = T(0);
+    __syncthreads();
+
+    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+    const LaneCols lc(P, lane);
+    int off[UM];
+#pragma unroll
+    for (int u = 0; u < UM; ++u) off[u] = lc.j(u) * tl.Szp + lc.k(u);
+    const int slab = tl.S * tl.Szp;
+    const T h = (T)g.h, c = (T)g.c, dn = (T)g.d;
+    const int beg = item_beg[item], end = item_end[item];
+    for (int p = beg + wave; p < end; p += nw) {
+        const Rec<T> r = rec[p];
+        T fx = 0, fy = 0, fz = 0;
+        if (lane < P) {
+            const T o = (T)(g.first + lane) * h;
+            const T ax = r.t[0] - o, ay = r.t[1] - o, az = r.t[2] - o;
+            fx = dexp(-c * ax * ax) * (r.q * dn);
+            fy = dexp(-c * ay * ay);
+            fz = dexp(-c * az * az);
+        }
+        T yz[UM];
+#pragma unroll
+        for (int u = 0; u < UM; ++u) {
+            T v = __shfl(fy, lc.j(u), 64) * __shfl(fz, lc.k(u), 64);
+            yz[u] = lc.ok(u) ? v : T(0);
+        }
+        const int lx = r.lo & 255, ly = (r.lo >> 8) & 255, lz = r.lo >> 16;
+        T* base = sub + (lx * tl.S + ly) * tl.Szp + lz;
+#pragma unroll 4
+        for (int i = 0; i < P; ++i) {
+            const T xi = bcast(fx, i);
+            T* row = base + i * slab;
+#pragma unroll
+            for (int u = 0; u < UM; ++u)
+                if (lc.ok(u)) lds_add(row + off[u], xi * yz[u]);
+        }
+    }
+    __syncthreads();
+    T* out = partials + (size_t)item * tl.part_elems;
+    const int S = tl.S;
+    for (int e = tid; e < tl.part_elems; e += nthr) {
+        int iz = e % S, r2 = e / S, iy = r2 % S, ix = r2 / S;
+        out[e] = sub[(ix * S + iy) * tl.Szp + iz];
+    }
+}
+
+template <typename T>
+void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg, const int* item_end,
+                         const int* n_items, const Geom& g, const Tiling& tl, T* partials, hipStream_t s)
+{
+    size_t lds = (size_t)tl.sub_elems * sizeof(T);
+    dim3 grid(tl.max_items), block(tl.block);
+#define SE_SPREAD(PT, UM)                                                                                   \
+    do {                                                                                                    \
+        allow_lds((const void*)k_spread_tiles<T, PT, UM>, lds);                                             \
+        hipLaunchKernelGGL((k_spread_tiles<T, PT, UM>), grid, block, lds, s, rec, item_tile, item_beg,      \
+                           item_end, n_items, g, tl, partials);                                             \
+    } while (0)
+    const int U = (g.P * g.P + 63) / 64;
+    switch (g.P) {
+    case 8: SE_SPREAD(8, 1); break;
+    case 11: SE_SPREAD(11, 2); break;
+    case 16: SE_SPREAD(16, 4); break;
+    case 21: SE_SPREAD(21, 7); break;
+    default:
+        if (U <= 1) SE_SPREAD(0, 1);
+        else if (U <= 2) SE_SPREAD(0, 2);
+        else if (U <= 4) SE_SPREAD(0, 4);
+        else if (U <= 8) SE_SPREAD(0, 8);
+        else if (U <= 12) SE_SPREAD(0, 12);
+        else SE_SPREAD(0, 16);
+    }
+#undef SE_SPREAD
+}
+
+// ================================================================================
+// Halo merge (= wrap): each periodic grid point sums, in a fixed order, the
+// partial values of every (tile, chunk) whose sub-grid covers it.  FP32 partials
+// are summed in FP64; FP64 partials with Kahan compensation.
+// cover[g*ncov + a] = (tile coordinate, local offset) pairs covering coordinate g.
+// ================================================================================
+template <typename T>
+struct MergeAcc;
+template <>
+struct MergeAcc<float> {
+    double s = 0.0;
+    __device__ void add(float v) { s += (double)v; }
+    __device__ float get() const { return (float)s; }
+};
+template <>
+struct MergeAcc<double> {
+    double s = 0.0, c = 0.0;
+    __device__ void add(double v)
+    {
+        double y = v - c;
+        double t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ double get() const { return s; }
+};
+
+template <typename T>
+__global__ void k_merge(const T* __restrict__ partials, const int* __restrict__ chunk_start,
+                        const int2* __restrict__ cover, const int* __restrict__ ncover, Geom g, Tiling tl,
+                        T* __restrict__ grid)
+{
+    const long M = g.M;
+    long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= M * M * M) return;
+    const int gz = (int)(gid % M);
+    const long r = gid / M;
+    const int gy = (int)(r % M), gx = (int)(r / M);
+    const int S = tl.S, NT = tl.NT, nc = tl.ncov;
+    const int cx = ncover[gx], cy = ncover[gy], cz = ncover[gz];
+    MergeAcc<T> acc;
+    for (int a = 0; a < cx; ++a) {
+        const int2 A = cover[gx * nc + a];
+        for (int b = 0; b < cy; ++b) {
+            const int2 B = cover[gy * nc + b];
+            for (int e = 0; e < cz; ++e) {
+                const int2 C = cover[gz * nc + e];
+                const int tile = (A.x * NT + B.x) * NT + C.x;
+                const int loc = (A.y * S + B.y) * S + C.y;
+                const int i1 = chunk_start[tile + 1];
+                for (int it = chunk_start[tile]; it < i1; ++it) acc.add(partials[(size_t)it * tl.part_elems + loc]);
+            }
+        }
+    }
+    grid[gid] = acc.get();
+}
+
+template <typename T>
+void launch_merge(const T* partials, const int* chunk_start, const int2* cover, const int* ncover, const Geom& g,
+                  const Tiling& tl, T* grid, hipStream_t s)
+{
+    long n = (long)g.M * g.M * g.M;
+    hipLaunchKernelGGL(k_merge<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, partials, chunk_start, cover, ncover, g,
+                       tl, grid);
+}
+
+// ================================================================================
+// Tiled gather (= extend + interpolate).  The work-group loads its tile's S^3
+// sub-grid from the periodic grid into LDS (modular indexing), then each wave
+// takes one particle: r_u = sum_i X_i G[i, j_u, k_u] per owned (j,k) pair,
+// phi_lane = sum_u Y_j Z_k r_u, and a wavefront reduction in FP64.  Force uses
+// the derivative factors zf = -c (t - o h) (SE_fgg.c:693-709):
+// F = h^3 q sum G W zf = (q/2) grad phi.
+// ================================================================================
+template <typename T, int PT, int UM, bool POT, bool FORCE>
+__global__ __launch_bounds__(512) void k_interp_tiles(
+    const Rec<T>* __restrict__ rec, const int* __restrict__ item_tile, const int* __restrict__ item_beg,
+    const int* __restrict__ item_end, const int* __restrict__ n_items, const T* __restrict__ grid, Geom g,
+    Tiling tl, int N, double* __restrict__ phi, double* __restrict__ force)
+{
+    const int item = blockIdx.x;
+    if (item >= *n_items) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* sub = reinterpret_cast<T*>(smem_raw);
+    const int P = PT > 0 ? PT : g.P;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int M = g.M, S = tl.S, NT = tl.NT;
+    {
+        const int tile = item_tile[item];
+        const int ox = (tile / (NT * NT)) * tl.TS, oy = ((tile / NT) % NT) * tl.TS, oz = (tile % NT) * tl.TS;
+        for (int e = tid; e < tl.part_elems; e += nthr) {
+            int iz = e % S, r2 = e / S, iy = r2 % S, ix = r2 / S;
+            int gx = (ox + ix) % M, gy = (oy + iy) % M, gz = (oz + iz) % M;
+            sub[(ix * S + iy) * tl.Szp + iz] = grid[((long)gx * M + gy) * M + gz];
+        }
+    }
+    __syncthreads();
+
+    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+    const LaneCols lc(P, lane);
+    int off[UM];
+#pragma unroll
+    for (int u = 0; u < UM; ++u) off[u] = lc.j(u) * tl.Szp + lc.k(u);
+    const int slab = S * tl.Szp;
+    const T h = (T)g.h, c = (T)g.c, dn = (T)g.d;
+    const double h3 = g.h * g.h * g.h;
+    const int beg = item_beg[item], end = item_end[item];
+    for (int p = beg + wave; p < end; p += nw) {
+        const Rec<T> r = rec[p];
+        T fx = 0, fy = 0, fz = 0, dx = 0, dy = 0, dz = 0;
+        if (lane < P) {
+            const T o = (T)(g.first + lane) * h;
+            const T ax = r.t[0] - o, ay = r.t[1] - o, az = r.t[2] - o;
+            fx = dexp(-c * ax * ax) * dn;
+            fy = dexp(-c * ay * ay);
+            fz = dexp(-c * az * az);
+            dx = -c * ax;
+            dy = -c * ay;
+            dz = -c * az;
+        }
+        T yz[UM];
+#pragma unroll
+        for (int u = 0; u < UM; ++u) {
+            T v = __shfl(fy, lc.j(u), 64) * __shfl(fz, lc.k(u), 64);
+            yz[u] = lc.ok(u) ? v : T(0);
+        }
+        const int lx = r.lo & 255, ly = (r.lo >> 8) & 255, lz = r.lo >> 16;
+        const T* base = sub + (lx * S + ly) * tl.Szp + lz;
+        T acc[UM], accf[UM];
+#pragma unroll
+        for (int u = 0; u < UM; ++u) acc[u] = accf[u] = T(0);
+        // FP64 with many columns per lane: no unrolling, or hoisted LDS loads spill
+        constexpr int UNR = (sizeof(T) == 8 && UM > 4) ? 1 : 4;
+#pragma unroll UNR
+        for (int i = 0; i < P; ++i) {
+            const T xi = bcast(fx, i);
+            const T xf = FORCE ? xi * bcast(dx, i) : T(0);
+            const T* row = base + i * slab;
+#pragma unroll
+            for (int u = 0; u < UM; ++u) {
+                if (lc.ok(u)) {
+                    const T sv = row[off[u]];
+                    acc[u] += xi * sv;
+                    if (FORCE) accf[u] += xf * sv;
+                }
+            }
+        }
+        double sp = 0, s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int u = 0; u < UM; ++u) {
+            const double w = (double)(yz[u] * acc[u]);
+            sp += w;
+            if (FORCE) {
+                s0 += (double)(yz[u] * accf[u]);
+                s1 += w * (double)__shfl(dy, lc.j(u), 64);
+                s2 += w * (double)__shfl(dz, lc.k(u), 64);
+            }
+        }
+        if (POT) sp = wave_sum(sp);
+        if (FORCE) {
+            s0 = wave_sum(s0);
+            s1 = wave_sum(s1);
+            s2 = wave_sum(s2);
+        }
+        if (lane == 0) {
+            const int o = r.orig;
+            if (POT) phi[o] = h3 * sp;
+            if (FORCE) {
+                const double qh = h3 * (double)r.q;
+                force[o] = qh * s0;
+                force[o + (long)N] = qh * s1;
+                force[o + 2l * N] = qh * s2;
+            }
+        }
+    }
+}
+
+template <typename T, int PT, int UM>
+static void interp_dispatch_mode(const Rec<T>* rec, const int* item_tile, const int* item_beg, const int* item_end,
+                                 const int* n_items, const T* grid, const Geom& g, const Tiling& tl, int N,
+                                 double* phi, double* force, hipStream_t s)
+{
+    size_t lds = (size_t)tl.sub_elems * sizeof(T);
+    dim3 gr(tl.max_items), bl(tl.block);
+#define SE_GATHER(POT, FORCE)                                                                                  \
+    do {                                                                                                       \
+        allow_lds((const void*)k_interp_tiles<T, PT, UM, POT, FORCE>, lds);                                    \
+        hipLaunchKernelGGL((k_interp_tiles<T, PT, UM, POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg,  \
+                           item_end, n_items, grid, g, tl, N, phi, force);                                     \
+    } while (0)
+    if (phi && force)
+        SE_GATHER(true, true);
+    else if (force)
+        SE_GATHER(false, true);
+    else
+        SE_GATHER(true, false);
+#undef SE_GATHER
+}
+
+template <typename T>
+void launch_interp_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg, const int* item_end,
+                         const int* n_items, const T* grid, const Geom& g, const Tiling& tl, int N, double* phi,
+                         double* force, hipStream_t s)
+{
+#define SE_GD(PT, UM) interp_dispatch_mode<T, PT, UM>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, \
+                                                     phi, force, s)
+    const int U = (g.P * g.P + 63) / 64;
+    switch (g.P) {
+    case 8: SE_GD(8, 1); break;
+    case 11: SE_GD(11, 2); break;
+    case 16: SE_GD(16, 4); break;
+    case 21: SE_GD(21, 7); break;
+    default:
+        if (U <= 1) SE_GD(0, 1);
+        else if (U <= 2) SE_GD(0, 2);
+        else if (U <= 4) SE_GD(0, 4);
+        else if (U <= 8) SE_GD(0, 8);
+        else if (U <= 12) SE_GD(0, 12);
+        else SE_GD(0, 16);
+    }
+#undef SE_GD
+}
+
+// ================================================================================
+// Green's function on the R2C half spectrum (SE_general.c:141-176 and
+// spectral_ewald.c:126-131): Z = 4pi/M^3 exp(s K^2)/K^2, Z(0) = 0, with
+// exp(s K^2) = prod_d exp(s k_d^2) from a per-axis table (no M^3 exp).
+// tab[0..M) = k^2, tab[M..2M) = exp(s k^2), tab[2M] = 4pi/M^3.
+// ================================================================================
+__global__ void k_green_tables(double* tab, int M, double L, double scalar)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) tab[2 * M] = 4.0 * M_PI / ((double)M * M * M);
+    if (i >= M) return;
+    const int pos = (M % 2 == 0) ? M / 2 : (M - 1) / 2 + 1;
+    double k = 2.0 * M_PI / L * (double)(i < pos ? i : i - M);
+    tab[i] = k * k;
+    tab[M + i] = exp(scalar * k * k);
+}
+
+void launch_green_tables(double* tab, int M, double L, double scalar, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_green_tables, dim3(cdiv(M, 256)), dim3(256), 0, s, tab, M, L, scalar);
+}
+
+template <typename T>
+__global__ void k_green(T* __restrict__ spec, int M, const double* __restrict__ tab)
+{
+    const int Mh = M / 2 + 1;
+    long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= (long)M * M * Mh) return;
+    const int kc = (int)(gid % Mh);
+    const long r = gid / Mh;
+    const int j = (int)(r % M), i = (int)(r / M);
+    const double K2 = (tab[i] + tab[j]) + tab[kc];
+    const double z = (K2 > 0.0) ? tab[M + i] * tab[M + j] * tab[M + kc] / K2 * tab[2 * M] : 0.0;
+    spec[2 * gid] = (T)((double)spec[2 * gid] * z);
+    spec[2 * gid + 1] = (T)((double)spec[2 * gid + 1] * z);
+}
+
+template <typename T>
+void launch_green(void* spec, int M, const double* tab, hipStream_t s)
+{
+    long n = (long)M * M * (M / 2 + 1);
+    hipLaunchKernelGGL(k_green<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, (T*)spec, M, tab);
+}
+
+// E = 1/2 sum phi q  (SE_fgg.c:5934-5946), FP64 block tree + one atomic per block.
+__global__ void k_energy(const double* __restrict__ phi, const double* __restrict__ q, int N, double* out)
+{
+    __shared__ double red[4];
+    double v = 0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+        v += phi[i] * q[i];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, 0.5 * (red[0] + red[1] + red[2] + red[3]));
+}
+
+__global__ void k_zero_d(double* p) { *p = 0.0; }
+
+void launch_energy(const double* phi, const double* q, int N, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(1), 0, s, out);
+    if (N <= 0) return;
+    int blocks = std::min(cdiv(N, 256), 1024);
+    hipLaunchKernelGGL(k_energy, dim3(blocks), dim3(256), 0, s, phi, q, N, out);
+}
+
+// ================================================================================
+// Reference-layout FP64 kernels (drop-in SE_FGG_* surface)
+// ================================================================================
+
+// SE_fgg.c:508-535
+__global__ void k_base_gaussian(LParams p, double* zs)
+{
+    const int P = p.P, n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P * P * P) return;
+    const int lo = (P & 1) ? p.P_half : p.P_half - 1;
+    const int i = n / (P * P) - lo, j = (n / P) % P - lo, k = n % P - lo;
+    const double h = p.h;
+    zs[n] = p.d * exp(-p.c * ((i * h) * (i * h) + (j * h) * (j * h) + (k * h) * (k * h)));
+}
+
+void launch_base_gaussian(const LParams& p, double* zs, hipStream_t s)
+{
+    int n = p.P * p.P * p.P;
+    hipLaunchKernelGGL(k_base_gaussian, dim3(cdiv(n, 256)), dim3(256), 0, s, p, zs);
+}
+
+// Centering of one coordinate (SE_fgg.c:554-573, 2P :741-768, 1P :838-865).
+__device__ void legacy_center(const LParams& p, int per, int d, double x, int& from, double& t)
+{
+    const bool freed = (per == 2 && d == 2) || (per == 1 && d < 2);
+    const int odd = p.P & 1, ph = p.P_half;
+    const double h = p.h;
+    int i;
+    if (freed) {
+        const double origin = ((per == 2) ? p.a : (d == 0 ? p.a : p.b)) + h / 2;
+        const double u = (x - origin) / h;
+        i = odd ? (int)round(u) : (int)floor(u);
+        t = x - (i * h + origin);
+    } else {
+        const double u = x / h;
+        i = odd ? (int)round(u) : (int)floor(u);
+        t = x - h * i;
+    }
+    from = odd ? i - ph : i - (ph - 1);
+}
+
+__device__ int legacy_index(const LParams& p, int per, const int from[3])
+{
+    const int ph = p.P_half;
+    const int e0 = from[0] + (per == 1 ? 0 : ph), e1 = from[1] + (per == 1 ? 0 : ph),
+              e2 = from[2] + (per == 2 ? 0 : ph);
+    return (e0 * p.npdims[1] + e1) * p.npdims[2] + e2;
+}
+
+// True when the whole P^3 window starting at extended index i0 lies inside the
+// extended grid.  The reference does not check (positions outside the box index
+// outside H); here such particles are skipped (spread) or get NaN (gather).
+__device__ __forceinline__ bool window_inside(const LParams& p, long i0)
+{
+    const long n1 = p.npdims[1], n2 = p.npdims[2];
+    const long last = i0 + ((long)(p.P - 1) * n1 + (p.P - 1)) * n2 + (p.P - 1);
+    return i0 >= 0 && last < (long)p.npdims[0] * n1 * n2;
+}
+
+// One thread per (particle, dimension): SE_fgg.c:539-722 (pow, then recurrence).
+__global__ void k_expand_all(LParams p, int force, const double* __restrict__ x, double* zx, double* zy, double* zz,
+                             int* idx, double* zfx, double* zfy, double* zfz)
+{
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int N = p.N, P = p.P;
+    if (gid >= 3l * N) return;
+    const int n = (int)(gid / 3), d = (int)(gid % 3);
+    const int per = (force && p.per == 1) ? 3 : p.per;   // SE_fgg.h:29-31
+    int from[3];
+    double t[3];
+    for (int e = 0; e < 3; ++e) legacy_center(p, per, e, x[n + (long)e * N], from[e], t[e]);
+    const double c = p.c, h = p.h;
+    const int first = (P & 1) ? -p.P_half : -p.P_half + 1;
+    const double z3 = exp(-c * (t[0] * t[0] + t[1] * t[1] + t[2] * t[2])) * 1.0;
+    const double base = exp(2 * c * h * t[d]);
+    double z = pow(base, (double)first);
+    double* out = (d == 0 ? zx : (d == 1 ? zy : zz)) + (long)n * P;
+    double* fo = force ? (d == 0 ? zfx : (d == 1 ? zfy : zfz)) + (long)n * P : nullptr;
+    for (int i = 0; i < P; ++i) {
+        if (i > 0) z *= base;
+        out[i] = (d == 0) ? z * z3 : z;
+        if (fo) fo[i] = -c * (t[d] - (first + i) * h);
+    }
+    if (d == 0) idx[n] = legacy_index(p, per, from);
+}
+
+void launch_expand_all(const LParams& p, int force, const double* x, double* zx, double* zy, double* zz, int* idx,
+                       double* zfx, double* zfy, double* zfz, hipStream_t s)
+{
+    if (p.N <= 0) return;
+    hipLaunchKernelGGL(k_expand_all, dim3(cdiv(3l * p.N, 256)), dim3(256), 0, s, p, force, x, zx, zy, zz, idx, zfx,
+                       zfy, zfz);
+}
+
+// SE_FGG_grid_split (SE_fgg.c:3983-4028): one wave per particle, global FP64 atomics.
+__global__ void k_grid_split_ext(LParams p, const double* __restrict__ q, const double* __restrict__ zx,
+                                 const double* __restrict__ zy, const double* __restrict__ zz,
+                                 const int* __restrict__ idx, const double* __restrict__ zs, double* H)
+{
+    const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (n >= p.N) return;
+    const int P = p.P, n1 = p.npdims[1], n2 = p.npdims[2];
+    const double qn = q[n];
+    const long i0 = idx[n];
+    if (!window_inside(p, i0)) return;
+    for (int pr = lane; pr < P * P; pr += 64) {
+        const int j = pr / P, k = pr % P;
+        const double zzk = zz[(long)n * P + k], zyj = zy[(long)n * P + j];
+        for (int i = 0; i < P; ++i) {
+            const double cij = qn * zx[(long)n * P + i] * zyj;
+            atomicAdd(H + i0 + ((long)i * n1 + j) * n2 + k, zs[(i * P + j) * P + k] * zzk * cij);
+        }
+    }
+}
+
+void launch_grid_split_ext(const LParams& p, const double* q, const double* zx, const double* zy, const double* zz,
+                           const int* idx, const double* zs, double* H, hipStream_t s)
+{
+    if (p.N <= 0) return;
+    hipLaunchKernelGGL(k_grid_split_ext, dim3(cdiv(p.N, 4)), dim3(256), 0, s, p, q, zx, zy, zz, idx, zs, H);
+}
+
+// Per-wave on-the-fly expansion into LDS: f[0..P) = zx (times qx), f[32..) = zy, f[64..) = zz,
+// f[96..) .. zf (force).  Returns the extended-grid index.
+__device__ int legacy_expand_wave(const LParams& p, int per, const double* x, long n, int N, double qx, int lane,
+                                  double* f, bool force)
+{
+    int from[3];
+    double t[3];
+    for (int e = 0; e < 3; ++e) legacy_center(p, per, e, x[n + (long)e * N], from[e], t[e]);
+    const int P = p.P;
+    const int first = (P & 1) ? -p.P_half : -p.P_half + 1;
+    const double c = p.c, h = p.h;
+    if (lane < P) {
+        const double z3 = exp(-c * (t[0] * t[0] + t[1] * t[1] + t[2] * t[2])) * qx;
+        for (int d = 0; d < 3; ++d) {
+            const double base = exp(2 * c * h * t[d]);
+            const double z = pow(base, (double)(first + lane));
+            f[32 * d + lane] = (d == 0) ? z * z3 : z;
+            if (force) f[96 + 32 * d + lane] = -c * (t[d] - (first + lane) * h);
+        }
+    }
+    __syncthreads();
+    return legacy_index(p, per, from);
+}
+
+// SE_FGG_grid (SE_fgg.c:3866-3914): one wave per block.
+__global__ void k_grid_ext(LParams p, const double* __restrict__ x, const double* __restrict__ q,
+                           const double* __restrict__ zs, double* H)
+{
+    __shared__ double f[192];
+    const int n = blockIdx.x, lane = threadIdx.x;
+    const int P = p.P, n1 = p.npdims[1], n2 = p.npdims[2];
+    const int i0 = legacy_expand_wave(p, p.per, x, n, p.N, q[n], lane, f, false);
+    if (!window_inside(p, i0)) return;
+    for (int pr = lane; pr < P * P; pr += 64) {
+        const int j = pr / P, k = pr % P;
+        for (int i = 0; i < P; ++i) {
+            const double cij = f[i] * f[32 + j];
+            atomicAdd(H + i0 + ((long)i * n1 + j) * n2 + k, zs[(i * P + j) * P + k] * f[64 + k] * cij);
+        }
+    }
+}
+
+void launch_grid_ext(const LParams& p, const double* x, const double* q, const double* zs, double* H, hipStream_t s)
+{
+    if (p.N <= 0) return;
+    hipLaunchKernelGGL(k_grid_ext, dim3(p.N), dim3(64), 0, s, p, x, q, zs, H);
+}
+
+// SE_FGG_int_split (SE_fgg.c:1176-1223) and _force (:2241-2312): one wave per particle.
+__global__ void k_int_split_ext(LParams p, int force, const double* __restrict__ H, const double* __restrict__ q,
+                                const double* __restrict__ zx, const double* __restrict__ zy,
+                                const double* __restrict__ zz, const double* __restrict__ zfx,
+                                const double* __restrict__ zfy, const double* __restrict__ zfz,
+                                const int* __restrict__ idx, const double* __restrict__ zs, double* out)
+{
+    const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (n >= p.N) return;
+    const int P = p.P, n1 = p.npdims[1], n2 = p.npdims[2];
+    const long i0 = idx[n];
+    const long b = (long)n * P;
+    const double qm = force ? q[n] : 1.0;
+    if (!window_inside(p, i0)) {
+        if (lane == 0) {
+            out[n] = NAN;
+            if (force) out[n + (long)p.N] = out[n + 2l * p.N] = NAN;
+        }
+        return;
+    }
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int pr = lane; pr < P * P; pr += 64) {
+        const int j = pr / P, k = pr % P;
+        for (int i = 0; i < P; ++i) {
+            const double cij = zx[b + i] * zy[b + j];
+            const double v = H[i0 + ((long)i * n1 + j) * n2 + k] * zs[(i * P + j) * P + k] * zz[b + k] * cij;
+            if (force) {
+                const double w = v * qm;
+                s0 += w * zfx[b + i];
+                s1 += w * zfy[b + j];
+                s2 += w * zfz[b + k];
+            } else {
+                s0 += v;
+            }
+        }
+    }
+    s0 = wave_sum(s0);
+    if (force) {
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+    }
+    if (lane == 0) {
+        const double h3 = p.h * p.h * p.h;
+        out[n] = h3 * s0;
+        if (force) {
+            out[n + (long)p.N] = h3 * s1;
+            out[n + 2l * p.N] = h3 * s2;
+        }
+    }
+}
+
+void launch_int_split_ext(const LParams& p, int force, const double* H, const double* q, const double* zx,
+                          const double* zy, const double* zz, const double* zfx, const double* zfy,
+                          const double* zfz, const int* idx, const double* zs, double* out, hipStream_t s)
+{
+    if (p.N <= 0) return;
+    hipLaunchKernelGGL(k_int_split_ext, dim3(cdiv(p.N, 4)), dim3(256), 0, s, p, force, H, q, zx, zy, zz, zfx, zfy,
+                       zfz, idx, zs, out);
+}
+
+// SE_FGG_int (SE_fgg.c:968-1019, q = 1) and SE_FGG_int_force (:1024-1100, q in the expansion).
+__global__ void k_int_ext(LParams p, int force, const double* __restrict__ H, const double* __restrict__ x,
+                          const double* __restrict__ q, const double* __restrict__ zs, double* out)
+{
+    __shared__ double f[192];
+    const int n = blockIdx.x, lane = threadIdx.x;
+    const int P = p.P, n1 = p.npdims[1], n2 = p.npdims[2];
+    const int per = (force && p.per == 1) ? 3 : p.per;
+    const long i0 = legacy_expand_wave(p, per, x, n, p.N, force ? q[n] : 1.0, lane, f, force);
+    if (!window_inside(p, i0)) {
+        if (lane == 0) {
+            out[n] = NAN;
+            if (force) out[n + (long)p.N] = out[n + 2l * p.N] = NAN;
+        }
+        return;
+    }
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int pr = lane; pr < P * P; pr += 64) {
+        const int j = pr / P, k = pr % P;
+        for (int i = 0; i < P; ++i) {
+            const double cij = f[i] * f[32 + j];
+            const double v = H[i0 + ((long)i * n1 + j) * n2 + k] * zs[(i * P + j) * P + k] * f[64 + k] * cij;
+            if (force) {
+                s0 += v * f[96 + i];
+                s1 += v * f[128 + j];
+                s2 += v * f[160 + k];
+            } else {
+                s0 += v;
+            }
+        }
+    }
+    s0 = wave_sum(s0);
+    if (force) {
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+    }
+    if (lane == 0) {
+        const double h3 = p.h * p.h * p.h;
+        out[n] = h3 * s0;
+        if (force) {
+            out[n + (long)p.N] = h3 * s1;
+            out[n + 2l * p.N] = h3 * s2;
+        }
+    }
+}
+
+void launch_int_ext(const LParams& p, int force, const double* H, const double* x, const double* q,
+                    const double* zs, double* out, hipStream_t s)
+{
+    if (p.N <= 0) return;
+    hipLaunchKernelGGL(k_int_ext, dim3(p.N), dim3(64), 0, s, p, force, H, x, q, zs, out);
+}
+
+// Wrap (SE_fgg.c:327-411) as a scatter with FP64 atomics; extend (:417-502) as a gather.
+__device__ __forceinline__ long legacy_per_index(const LParams& p, long e)
+{
+    const int k = (int)(e % p.npdims[2]);
+    const long r = e / p.npdims[2];
+    const int j = (int)(r % p.npdims[1]), i = (int)(r / p.npdims[1]);
+    const int ph = p.P_half;
+    const int a = (p.per == 1) ? i : imod(i - ph, p.dims[0]);
+    const int b = (p.per == 1) ? j : imod(j - ph, p.dims[1]);
+    const int c = (p.per == 2) ? k : imod(k - ph, p.dims[2]);
+    return (long)a + (long)b * p.dims[0] + (long)c * p.dims[0] * p.dims[1];
+}
+
+__global__ void k_wrap(LParams p, const double* __restrict__ H, double* Hper)
+{
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)p.npdims[0] * p.npdims[1] * p.npdims[2]) return;
+    atomicAdd(Hper + legacy_per_index(p, e), H[e]);
+}
+
+__global__ void k_extend(LParams p, const double* __restrict__ Hper, double* __restrict__ H)
+{
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)p.npdims[0] * p.npdims[1] * p.npdims[2]) return;
+    H[e] = Hper[legacy_per_index(p, e)];
+}
+
+void launch_wrap(const LParams& p, const double* H, double* Hper, hipStream_t s)
+{
+    long n = (long)p.npdims[0] * p.npdims[1] * p.npdims[2];
+    hipLaunchKernelGGL(k_wrap, dim3(cdiv(n, 256)), dim3(256), 0, s, p, H, Hper);
+}
+
+void launch_extend(const LParams& p, const double* Hper, double* H, hipStream_t s)
+{
+    long n = (long)p.npdims[0] * p.npdims[1] * p.npdims[2];
+    hipLaunchKernelGGL(k_extend, dim3(cdiv(n, 256)), dim3(256), 0, s, p, Hper, H);
+}
+
+// scaling (SE_general.c:141-176) on a cubic n^3 grid, reference index order.
+__global__ void k_scaling(double scalar, double* Z, int n, double L)
+{
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)n * n * n) return;
+    const int k = (int)(e % n);
+    const long r = e / n;
+    const int j = (int)(r % n), i = (int)(r / n);
+    const int pos = (n % 2 == 0) ? n / 2 : (n - 1) / 2 + 1;
+    const double f = 2.0 * M_PI / L;
+    const double k1 = f * (i < pos ? i : i - n), k2 = f * (j < pos ? j : j - n), k3 = f * (k < pos ? k : k - n);
+    const double K2 = (k1 * k1 + k2 * k2) + k3 * k3;
+    const double d = 4.0 * M_PI / (double)((long)n * n * n);
+    Z[e] = exp(scalar * K2) / K2 * d;
+}
+
+void launch_scaling(double scalar, double* Z, int n, double L, hipStream_t s)
+{
+    long m = (long)n * n * n;
+    hipLaunchKernelGGL(k_scaling, dim3(cdiv(m, 256)), dim3(256), 0, s, scalar, Z, n, L);
+}
+
+// product_rc (SE_general.c:336-362)
+__global__ void k_product_rc(const double* __restrict__ a, const double* __restrict__ b, double* c, long n, int flag)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double re = a[2 * i], im = a[2 * i + 1], s = b[i];
+    if (flag == 1) {
+        c[2 * i] = re * s;
+        c[2 * i + 1] = im * s;
+    } else {
+        c[2 * i] = re / s;
+        c[2 * i + 1] = im / s;
+    }
+}
+
+void launch_product_rc(const double* a, const double* b, double* c, long n, int flag, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_product_rc, dim3(cdiv(n, 256)), dim3(256), 0, s, a, b, c, n, flag);
+}
+
+__global__ void k_reorder_gather(const double* __restrict__ xin, const double* __restrict__ qin,
+                                 const int* __restrict__ perm, int N, double* xout, double* qout)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int j = perm[i];
+    xout[i] = xin[j];
+    xout[i + (long)N] = xin[j + (long)N];
+    xout[i + 2l * N] = xin[j + 2l * N];
+    qout[i] = qin[j];
+}
+
+void launch_reorder_gather(const double* xin, const double* qin, const int* perm, int N, double* xout,
+                           double* qout, hipStream_t s)
+{
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_reorder_gather, dim3(cdiv(N, 256)), dim3(256), 0, s, xin, qin, perm, N, xout, qout);
+}
+
+// ---- explicit instantiations ----------------------------------------------------
+#define SE_INST(T)                                                                                               \
+    template void launch_bin_keys<T>(const double*, int, const Geom&, const Tiling&, int*, int*, int*,          \
+                                     hipStream_t);                                                               \
+    template void launch_scatter<T>(const double*, const double*, int, const Geom&, const Tiling&, const int*,  \
+                                    const int*, const int*, Rec<T>*, hipStream_t);                               \
+    template void launch_spread_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \
+                                         const Geom&, const Tiling&, T*, hipStream_t);                           \
+    template void launch_merge<T>(const T*, const int*, const int2*, const int*, const Geom&, const Tiling&, T*, \
+                                  hipStream_t);                                                                  \
+    template void launch_interp_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \
+                                         const T*, const Geom&, const Tiling&, int, double*, double*,            \
+                                         hipStream_t);                                                           \
+    template void launch_green<T>(void*, int, const double*, hipStream_t);
+SE_INST(float)
+SE_INST(double)
+
+}  // namespace se

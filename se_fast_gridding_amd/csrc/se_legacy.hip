@@ -1,0 +1,798 @@
+// Drop-in SE_FGG_* / SE_general / spectral_ewald surface (se_fgg.h, se_general.h).
+// Host arrays in and out, as in the reference; the arithmetic runs in the
+// reference-layout FP64 kernels or, for spectral_ewald, the fused engine.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <sys/time.h>
+#include <vector>
+
+#define SE_FGG_NO_INLINE_HELPERS
+#include "../../include/se_general.h"
+#include "../../include/se_gpu.h"
+#include "se_internal.h"
+
+using namespace se;
+
+namespace {
+
+int fgg_half(int p) { return (p & 1) ? (p - 1) / 2 : p / 2; }   // SE_fgg.h:94-97
+
+std::recursive_mutex g_mu;
+int g_periodicity = 3;
+int g_quantity = SE_POTENTIAL;
+int g_precision = SE_FP64;
+hipStream_t g_stream = nullptr;
+
+// Grow-only device scratch slots for the synchronous legacy calls.
+struct Pool {
+    std::vector<void*> p;
+    std::vector<size_t> n;
+    void* get(int slot, size_t bytes)
+    {
+        if ((int)p.size() <= slot) {
+            p.resize(slot + 1, nullptr);
+            n.resize(slot + 1, 0);
+        }
+        if (n[slot] < bytes) {
+            if (p[slot]) SE_HIP(hipFree(p[slot]));
+            p[slot] = nullptr;
+            SE_HIP(hipMalloc(&p[slot], std::max<size_t>(bytes, 64)));
+            n[slot] = std::max<size_t>(bytes, 64);
+        }
+        return p[slot];
+    }
+} g_pool;
+
+hipStream_t stream()
+{
+    if (!g_stream) SE_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    return g_stream;
+}
+
+template <typename F>
+int run(const char* who, F&& f)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    try {
+        f();
+        return 0;
+    } catch (const se::Error& e) {
+        se::set_error(std::string(who) + ": " + e.msg);
+        std::fprintf(stderr, "[se_fgg_amd] %s: %s\n", who, e.msg.c_str());
+        return e.code ? e.code : -1;
+    }
+}
+
+LParams lp(const SE_FGG_params* p)
+{
+    LParams l{};
+    l.N = p->N;
+    l.P = p->P;
+    l.P_half = p->P_half;
+    for (int d = 0; d < 3; ++d) {
+        l.dims[d] = p->dims[d];
+        l.npdims[d] = p->npdims[d];
+    }
+    l.c = p->c;
+    l.d = p->d;
+    l.h = p->h;
+    l.a = p->a;
+    l.b = p->b;
+    l.per = g_periodicity;
+    SE_REQUIRE(l.P >= 1 && l.P <= P_MAX, SE_ERR_ARG, "P must be in 1..32");
+    return l;
+}
+
+size_t next_elems(const SE_FGG_params* p) { return (size_t)p->npdims[0] * p->npdims[1] * p->npdims[2]; }
+
+// Periodic-grid extent as the reference's wrap/extend index it (see se_fgg.h).
+size_t per_elems(const SE_FGG_params* p, int per)
+{
+    const int *d = p->dims, *e = p->npdims;
+    if (per == 3) return (size_t)d[0] * d[1] * d[2];
+    if (per == 2) return (size_t)d[0] * d[1] * e[2];
+    return (size_t)(e[0] - 1) + (size_t)(e[1] - 1) * d[0] + (size_t)(d[2] - 1) * d[0] * d[1] + 1;
+}
+
+template <typename T>
+T* up(int slot, const T* h, size_t n)
+{
+    T* d = (T*)g_pool.get(slot, n * sizeof(T));
+    if (n) SE_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream()));
+    return d;
+}
+
+template <typename T>
+T* dev(int slot, size_t n)
+{
+    return (T*)g_pool.get(slot, n * sizeof(T));
+}
+
+template <typename T>
+void down(T* h, const T* d, size_t n)
+{
+    if (n) SE_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, stream()));
+}
+
+void finish()
+{
+    SE_HIP(hipGetLastError());
+    SE_HIP(hipStreamSynchronize(stream()));
+}
+
+void* amalloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (posix_memalign(&p, 64, std::max<size_t>(bytes, 64)) != 0) return nullptr;
+    return p;
+}
+
+// --- shared bodies --------------------------------------------------------------
+void spread_split(SE_FGG_work* w, const SE_state* st, const SE_FGG_params* p)
+{
+    LParams l = lp(p);
+    const size_t NP = (size_t)l.N * l.P, ne = next_elems(p);
+    double* H = up(0, w->H, ne);
+    double* zx = up(1, w->zx, NP);
+    double* zy = up(2, w->zy, NP);
+    double* zz = up(3, w->zz, NP);
+    int* idx = up(4, w->idx, (size_t)l.N);
+    double* q = up(5, st->q, (size_t)l.N);
+    double* zs = up(6, w->zs, (size_t)l.P * l.P * l.P);
+    launch_grid_split_ext(l, q, zx, zy, zz, idx, zs, H, stream());
+    down(w->H, H, ne);
+    finish();
+}
+
+void int_split(double* phi, const SE_FGG_work* w, const SE_FGG_params* p, bool accumulate)
+{
+    LParams l = lp(p);
+    const size_t NP = (size_t)l.N * l.P, ne = next_elems(p);
+    double* H = up(0, w->H, ne);
+    double* zx = up(1, w->zx, NP);
+    double* zy = up(2, w->zy, NP);
+    double* zz = up(3, w->zz, NP);
+    int* idx = up(4, w->idx, (size_t)l.N);
+    double* zs = up(6, w->zs, (size_t)l.P * l.P * l.P);
+    double* out = dev<double>(7, (size_t)l.N);
+    launch_int_split_ext(l, 0, H, nullptr, zx, zy, zz, nullptr, nullptr, nullptr, idx, zs, out, stream());
+    std::vector<double> tmp;
+    if (accumulate) {
+        tmp.resize(l.N);
+        down(tmp.data(), out, (size_t)l.N);
+        finish();
+        for (int m = 0; m < l.N; ++m) phi[m] += tmp[m];
+    } else {
+        down(phi, out, (size_t)l.N);
+        finish();
+    }
+}
+
+void int_split_force(double* F, const SE_state* st, const SE_FGG_work* w, const SE_FGG_params* p)
+{
+    LParams l = lp(p);
+    const size_t NP = (size_t)l.N * l.P, ne = next_elems(p);
+    double* H = up(0, w->H, ne);
+    double* zx = up(1, w->zx, NP);
+    double* zy = up(2, w->zy, NP);
+    double* zz = up(3, w->zz, NP);
+    int* idx = up(4, w->idx, (size_t)l.N);
+    double* q = up(5, st->q, (size_t)l.N);
+    double* zs = up(6, w->zs, (size_t)l.P * l.P * l.P);
+    double* zfx = up(8, w->zfx, NP);
+    double* zfy = up(9, w->zfy, NP);
+    double* zfz = up(10, w->zfz, NP);
+    double* out = dev<double>(7, 3 * (size_t)l.N);
+    launch_int_split_ext(l, 1, H, q, zx, zy, zz, zfx, zfy, zfz, idx, zs, out, stream());
+    down(F, out, 3 * (size_t)l.N);
+    finish();
+}
+
+// The reference's AVX dispatcher sends P % 4 == 0, P % 8 != 0 (with increments divisible
+// by 4) to SE_FGG_int_split_AVX, which adds into phi (SE_fgg.c:1856).
+bool avx_dispatch_accumulates(const SE_FGG_params* p)
+{
+    const int P = p->P, incrj = p->dims[2], incri = p->npdims[2] * p->dims[1];
+    if ((P & 3) || (incri & 3) || (incrj & 3)) return false;
+    return P != 16 && P != 8 && (P % 8) != 0;
+}
+
+struct PlanKey {
+    int M, P, prec, cap;
+    double L, xi, m;
+};
+se_plan* g_plan = nullptr;
+PlanKey g_key{};
+
+se_plan* driver_plan(const SE_opt& o, double xi, int N, int prec)
+{
+    PlanKey k{o.M, o.P, prec, std::max(N, 1), o.box[0], xi, o.m};
+    if (g_plan && g_key.M == k.M && g_key.P == k.P && g_key.prec == k.prec && g_key.L == k.L && g_key.xi == k.xi &&
+        g_key.m == k.m && g_key.cap >= N)
+        return g_plan;
+    if (g_plan) {
+        se_plan_destroy(g_plan);
+        g_plan = nullptr;
+    }
+    se_plan_desc d{};
+    d.periodicity = 3;
+    d.M = o.M;
+    d.P = o.P;
+    d.box = o.box[0];
+    d.xi = xi;
+    d.m = o.m;
+    d.precision = prec == SE_FP32 ? SE_PREC_FP32 : SE_PREC_FP64;
+    d.n_max = std::max(N, 1);
+    int rc = se_plan_create(&g_plan, &d, stream());
+    if (rc != SE_OK) throw se::Error{rc, se_gpu_last_error()};
+    g_key = k;
+    return g_plan;
+}
+
+}  // namespace
+
+extern "C" {
+
+void se_fgg_set_periodicity(int per)
+{
+    if (per == 1 || per == 2 || per == 3) g_periodicity = per;
+}
+int se_fgg_get_periodicity(void) { return g_periodicity; }
+void se_set_quantity(int q)
+{
+    if (q >= 1 && q <= 3) g_quantity = q;
+}
+void se_set_precision(int p) { g_precision = (p == SE_FP32) ? SE_FP32 : SE_FP64; }
+
+// ---- parameters (SE_fgg.c:69-132) ----------------------------------------------
+static void pack(SE_FGG_params* p, int N, int M0, int M1, int M2, int P, double c, double h, double a, double b)
+{
+    p->N = N;
+    p->P = P;
+    p->P_half = fgg_half(P);
+    p->c = c;
+    p->d = std::pow(c / PI, 1.5);
+    p->h = h;
+    p->a = a;
+    p->b = b;
+    p->dims[0] = M0;
+    p->dims[1] = M1;
+    p->dims[2] = M2;
+    p->npdims[0] = M0 + P;
+    p->npdims[1] = M1 + P;
+    p->npdims[2] = M2 + P;
+}
+void SE_FGG_pack_params(SE_FGG_params* p, int N, int M0, int M1, int M2, int P, double c, double h)
+{
+    pack(p, N, M0, M1, M2, P, c, h, -1, -1);
+}
+void SE2P_FGG_pack_params(SE_FGG_params* p, int N, int M0, int M1, int M2, int P, double c, double h, double a)
+{
+    pack(p, N, M0, M1, M2, P, c, h, a, -1);
+}
+void SE1P_FGG_pack_params(SE_FGG_params* p, int N, int M0, int M1, int M2, int P, double c, double h, double a,
+                          double b)
+{
+    pack(p, N, M0, M1, M2, P, c, h, a, b);
+}
+
+// ---- workspace (SE_fgg.c:135-263): host arrays, as in the reference --------------
+static void alloc_ws(SE_FGG_work* w, const SE_FGG_params* p, int zs, int expa, bool force)
+{
+    const size_t ne = next_elems(p);
+    w->H = (double*)amalloc(ne * sizeof(double));
+    std::memset(w->H, 0, ne * sizeof(double));
+    w->zs = zs ? (double*)amalloc(sizeof(double) * p->P * p->P * p->P) : nullptr;
+    w->free_zs = zs;
+    const size_t NP = (size_t)p->N * p->P;
+    if (expa) {
+        w->zx = (double*)amalloc(NP * sizeof(double));
+        w->zy = (double*)amalloc(NP * sizeof(double));
+        w->zz = (double*)amalloc(NP * sizeof(double));
+        w->idx = (int*)amalloc(p->N * sizeof(int));
+        if (force) {
+            w->zfx = (double*)amalloc(NP * sizeof(double));
+            w->zfy = (double*)amalloc(NP * sizeof(double));
+            w->zfz = (double*)amalloc(NP * sizeof(double));
+        }
+    } else {
+        w->zx = w->zy = w->zz = nullptr;
+        w->idx = nullptr;
+        if (force) w->zfx = w->zfy = w->zfz = nullptr;
+    }
+    w->free_fgg_expa = expa;
+}
+void SE_FGG_allocate_workspace(SE_FGG_work* w, const SE_FGG_params* p, int zs, int expa)
+{
+    alloc_ws(w, p, zs, expa, false);
+}
+void SE_FGG_allocate_workspace_SSE_force(SE_FGG_work* w, const SE_FGG_params* p, int zs, int expa)
+{
+    alloc_ws(w, p, zs, expa, true);
+}
+double* SE_FGG_allocate_grid(const SE_FGG_params* p)
+{
+    // room for the 2P/1P index ranges too (a larger zeroed buffer is indistinguishable for 3P)
+    size_t n = std::max(per_elems(p, 3), per_elems(p, g_periodicity));
+    double* H = (double*)amalloc(n * sizeof(double));
+    std::memset(H, 0, n * sizeof(double));
+    return H;
+}
+double* SE_FGG_allocate_vec(int n)
+{
+    double* v = (double*)amalloc((size_t)n * sizeof(double));
+    std::memset(v, 0, (size_t)n * sizeof(double));
+    return v;
+}
+void SE_FGG_free_workspace(SE_FGG_work* w)
+{
+    free(w->H);
+    if (w->free_zs) free(w->zs);
+    if (w->free_fgg_expa) {
+        free(w->zx);
+        free(w->zy);
+        free(w->zz);
+        free(w->idx);
+    }
+}
+void SE_FGG_free_workspace_SSE_force(SE_FGG_work* w)
+{
+    SE_FGG_free_workspace(w);
+    if (w->free_fgg_expa) {
+        free(w->zfx);
+        free(w->zfy);
+        free(w->zfz);
+    }
+}
+
+// ---- expansion / base gaussian --------------------------------------------------
+void SE_FGG_base_gaussian(SE_FGG_work* w, const SE_FGG_params* p)
+{
+    run("SE_FGG_base_gaussian", [&] {
+        LParams l = lp(p);
+        const size_t n = (size_t)l.P * l.P * l.P;
+        double* zs = dev<double>(6, n);
+        launch_base_gaussian(l, zs, stream());
+        down(w->zs, zs, n);
+        finish();
+    });
+}
+
+static void expand(SE_FGG_work* w, const SE_state* st, const SE_FGG_params* p, int force, const char* who)
+{
+    run(who, [&] {
+        LParams l = lp(p);
+        const size_t NP = (size_t)l.N * l.P;
+        double* x = up(11, st->x, 3 * (size_t)l.N);
+        double* zx = dev<double>(1, NP);
+        double* zy = dev<double>(2, NP);
+        double* zz = dev<double>(3, NP);
+        int* idx = dev<int>(4, (size_t)l.N);
+        double* zfx = force ? dev<double>(8, NP) : nullptr;
+        double* zfy = force ? dev<double>(9, NP) : nullptr;
+        double* zfz = force ? dev<double>(10, NP) : nullptr;
+        launch_expand_all(l, force, x, zx, zy, zz, idx, zfx, zfy, zfz, stream());
+        down(w->zx, zx, NP);
+        down(w->zy, zy, NP);
+        down(w->zz, zz, NP);
+        down(w->idx, idx, (size_t)l.N);
+        if (force) {
+            down(w->zfx, zfx, NP);
+            down(w->zfy, zfy, NP);
+            down(w->zfz, zfz, NP);
+        }
+        finish();
+    });
+}
+void SE_FGG_expand_all(SE_FGG_work* w, const SE_state* st, const SE_FGG_params* p)
+{
+    expand(w, st, p, 0, "SE_FGG_expand_all");
+}
+void SE_FGG_expand_all_SSE_force(SE_FGG_work* w, const SE_state* st, const SE_FGG_params* p)
+{
+    expand(w, st, p, 1, "SE_FGG_expand_all_SSE_force");
+}
+
+// ---- spread -------------------------------------------------------------------
+void SE_FGG_grid(SE_FGG_work* w, const SE_state* st, const SE_FGG_params* p)
+{
+    run("SE_FGG_grid", [&] {
+        LParams l = lp(p);
+        const size_t ne = next_elems(p);
+        double* H = up(0, w->H, ne);
+        double* x = up(11, st->x, 3 * (size_t)l.N);
+        double* q = up(5, st->q, (size_t)l.N);
+        double* zs = up(6, w->zs, (size_t)l.P * l.P * l.P);
+        launch_grid_ext(l, x, q, zs, H, stream());
+        down(w->H, H, ne);
+        finish();
+    });
+}
+
+#define SE_SPREAD_SPLIT(name)                                                           \
+    void name(SE_FGG_work* w, const SE_state* st, const SE_FGG_params* p)              \
+    {                                                                                   \
+        run(#name, [&] { spread_split(w, st, p); });                                    \
+    }
+SE_SPREAD_SPLIT(SE_FGG_grid_split_SSE_dispatch)
+SE_SPREAD_SPLIT(SE_FGG_grid_split)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_SSE)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_SSE_u8)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_SSE_P16)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX_dispatch)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX_P16)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX_P8)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX_u8)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_SSE_dispatch_force)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_force)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_SSE_force)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_SSE_u8_force)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_SSE_P16_force)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX_dispatch_force)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX_P16_force)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX_P8_force)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX_u8_force)
+SE_SPREAD_SPLIT(SE_FGG_grid_split_AVX_force)
+
+// ---- interpolation ------------------------------------------------------------
+void SE_FGG_int(double* phi, const SE_FGG_work* w, const SE_state* st, const SE_FGG_params* p)
+{
+    run("SE_FGG_int", [&] {
+        LParams l = lp(p);
+        const size_t ne = next_elems(p);
+        double* H = up(0, w->H, ne);
+        double* x = up(11, st->x, 3 * (size_t)l.N);
+        double* zs = up(6, w->zs, (size_t)l.P * l.P * l.P);
+        double* out = dev<double>(7, (size_t)l.N);
+        launch_int_ext(l, 0, H, x, nullptr, zs, out, stream());
+        down(phi, out, (size_t)l.N);
+        finish();
+    });
+}
+
+void SE_FGG_int_force(double* F, const SE_FGG_work* w, SE_state* st, const SE_FGG_params* p)
+{
+    run("SE_FGG_int_force", [&] {
+        LParams l = lp(p);
+        const size_t ne = next_elems(p);
+        double* H = up(0, w->H, ne);
+        double* x = up(11, st->x, 3 * (size_t)l.N);
+        double* q = up(5, st->q, (size_t)l.N);
+        double* zs = up(6, w->zs, (size_t)l.P * l.P * l.P);
+        double* out = dev<double>(7, 3 * (size_t)l.N);
+        launch_int_ext(l, 1, H, x, q, zs, out, stream());
+        down(F, out, 3 * (size_t)l.N);
+        finish();
+    });
+}
+
+#define SE_INT_SPLIT(name, acc)                                                             \
+    void name(double* phi, const SE_FGG_work* w, const SE_FGG_params* p)                   \
+    {                                                                                       \
+        run(#name, [&] { int_split(phi, w, p, acc); });                                     \
+    }
+SE_INT_SPLIT(SE_FGG_int_split_SSE_dispatch, false)
+SE_INT_SPLIT(SE_FGG_int_split, false)
+SE_INT_SPLIT(SE_FGG_int_split_SSE, false)
+SE_INT_SPLIT(SE_FGG_int_split_SSE_u8, false)
+SE_INT_SPLIT(SE_FGG_int_split_SSE_P8, false)
+SE_INT_SPLIT(SE_FGG_int_split_SSE_P16, false)
+SE_INT_SPLIT(SE_FGG_int_split_AVX_dispatch, avx_dispatch_accumulates(p))
+SE_INT_SPLIT(SE_FGG_int_split_AVX, true)   // adds into phi, as SE_fgg.c:1856
+SE_INT_SPLIT(SE_FGG_int_split_AVX_P8, false)
+SE_INT_SPLIT(SE_FGG_int_split_AVX_P16, false)
+SE_INT_SPLIT(SE_FGG_int_split_AVX_u8, false)
+
+#define SE_INT_SPLIT_FORCE(name)                                                              \
+    void name(double* F, SE_state* st, const SE_FGG_work* w, const SE_FGG_params* p)         \
+    {                                                                                         \
+        run(#name, [&] { int_split_force(F, st, w, p); });                                    \
+    }
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_SSE_dispatch_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_SSE_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_SSE_P8_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_SSE_u8_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_SSE_P16_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_AVX_dispatch_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_AVX_P16_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_AVX_P8_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_AVX_u8_force)
+SE_INT_SPLIT_FORCE(SE_FGG_int_split_AVX_force)
+
+// ---- wrap / extend ------------------------------------------------------------
+static void wrap_impl(double* Hper, const SE_FGG_work* w, const SE_FGG_params* p, int per, const char* who)
+{
+    run(who, [&] {
+        LParams l = lp(p);
+        l.per = per;
+        const size_t ne = next_elems(p), np = per_elems(p, per);
+        double* H = up(0, w->H, ne);
+        double* Hp = up(12, Hper, np);
+        launch_wrap(l, H, Hp, stream());
+        down(Hper, Hp, np);
+        finish();
+    });
+}
+static void extend_impl(SE_FGG_work* w, const double* Hper, const SE_FGG_params* p, int per, const char* who)
+{
+    run(who, [&] {
+        LParams l = lp(p);
+        l.per = per;
+        const size_t ne = next_elems(p), np = per_elems(p, per);
+        double* Hp = up(12, Hper, np);
+        double* H = dev<double>(0, ne);
+        launch_extend(l, Hp, H, stream());
+        down(w->H, H, ne);
+        finish();
+    });
+}
+void SE_FGG_wrap_fcn(double* Hper, const SE_FGG_work* w, const SE_FGG_params* p)
+{
+    wrap_impl(Hper, w, p, 3, "SE_FGG_wrap_fcn");
+}
+void SE2P_FGG_wrap_fcn(double* Hper, const SE_FGG_work* w, const SE_FGG_params* p)
+{
+    wrap_impl(Hper, w, p, 2, "SE2P_FGG_wrap_fcn");
+}
+void SE1P_FGG_wrap_fcn(double* Hper, const SE_FGG_work* w, const SE_FGG_params* p)
+{
+    wrap_impl(Hper, w, p, 1, "SE1P_FGG_wrap_fcn");
+}
+void SE_FGG_extend_fcn(SE_FGG_work* w, const double* Hper, const SE_FGG_params* p)
+{
+    extend_impl(w, Hper, p, 3, "SE_FGG_extend_fcn");
+}
+void SE2P_FGG_extend_fcn(SE_FGG_work* w, const double* Hper, const SE_FGG_params* p)
+{
+    extend_impl(w, Hper, p, 2, "SE2P_FGG_extend_fcn");
+}
+void SE1P_FGG_extend_fcn(SE_FGG_work* w, const double* Hper, const SE_FGG_params* p)
+{
+    extend_impl(w, Hper, p, 1, "SE1P_FGG_extend_fcn");
+}
+
+// ---- systems and utilities (host-side, as in the reference) --------------------
+static double randnum(double lo, double L) { return L * ((double)rand() / RAND_MAX) + lo; }
+
+void SE_init_system(SE_state* s, const SE_FGG_params* p)
+{
+    const int N = p->N;
+    s->x = (double*)amalloc(3 * (size_t)N * sizeof(double));
+    s->q = (double*)amalloc((size_t)N * sizeof(double));
+    s->phi = (double*)amalloc((size_t)N * sizeof(double));
+    for (int i = 0; i < N; i++) {
+        s->q[i] = randnum(-1, 2);
+        s->x[i] = randnum(0, 1);
+        s->x[i + N] = randnum(0, 1);
+        s->x[i + 2 * N] = randnum(0, 1);
+    }
+}
+
+void SE_init_unit_system(SE_state* s, const SE_FGG_params* p)
+{
+    const int N = p->N;
+    if (N % 2 != 0) return;   // SE_fgg.c:270-271
+    s->x = (double*)amalloc(3 * (size_t)N * sizeof(double));
+    s->q = (double*)amalloc((size_t)N * sizeof(double));
+    s->phi = (double*)amalloc((size_t)N * sizeof(double));
+    FILE* fp = std::fopen("atoms.txt", "r");
+    if (!fp) {
+        std::printf("Not successful!!!\n");
+        return;
+    }
+    int err = 0;
+    for (int i = 0; i < N; i++) {
+        err = std::fscanf(fp, "%lf", &s->x[i]);
+        err |= std::fscanf(fp, "%lf", &s->x[i + N]);
+        err |= std::fscanf(fp, "%lf", &s->x[i + 2 * N]);
+        err |= std::fscanf(fp, "%lf", &s->q[i]);
+    }
+    std::fclose(fp);
+    if (err == EOF) std::printf("Not successful!!!\n");
+}
+
+void SE_free_system(SE_state* s)
+{
+    free(s->x);
+    free(s->q);
+}
+
+double SE_gettime(void)
+{
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+int SE_prod3(const int v[3]) { return v[0] * v[1] * v[2]; }
+
+void SE_fp_set_zero(double* x, int N)
+{
+    if (N > 0) std::memset(x, 0, (size_t)N * sizeof(double));
+}
+
+void SE_FGG_reorder_system(SE_state* s, const SE_FGG_work* w, const SE_FGG_params* p)
+{
+    run("SE_FGG_reorder_system", [&] {
+        const int N = p->N;
+        if (N <= 0) return;
+        int* keys = up(4, w->idx, (size_t)N);
+        int* keys_out = dev<int>(13, (size_t)N);
+        int* vals = dev<int>(14, (size_t)N);
+        int* perm = dev<int>(15, (size_t)N);
+        std::vector<int> iota(N);
+        for (int i = 0; i < N; ++i) iota[i] = i;
+        SE_HIP(hipMemcpyAsync(vals, iota.data(), N * sizeof(int), hipMemcpyHostToDevice, stream()));
+        size_t tb = 0;
+        SE_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, keys_out, vals, perm, N, 0, 32, stream()));
+        void* tmp = g_pool.get(16, tb);
+        SE_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, keys_out, vals, perm, N, 0, 32, stream()));
+        double* x = up(11, s->x, 3 * (size_t)N);
+        double* q = up(5, s->q, (size_t)N);
+        double* xo = dev<double>(17, 3 * (size_t)N);
+        double* qo = dev<double>(18, (size_t)N);
+        launch_reorder_gather(x, q, perm, N, xo, qo, stream());
+        down(s->x, xo, 3 * (size_t)N);
+        down(s->q, qo, (size_t)N);
+        finish();
+    });
+}
+
+double calc_energy(SE_state st, int N)
+{
+    double e = 0.0;
+    run("calc_energy", [&] {
+        double* phi = up(7, st.phi, (size_t)N);
+        double* q = up(5, st.q, (size_t)N);
+        double* out = dev<double>(19, 1);
+        launch_energy(phi, q, N, out, stream());
+        down(&e, out, 1);
+        finish();
+    });
+    return e;
+}
+
+// ---- SE_general ----------------------------------------------------------------
+void parse_params(SE_opt* opt, double xi)
+{
+    const double h0 = opt->box[0] / (double)opt->M;
+    const double w0 = (double)opt->P * h0 / 2.;
+    const double eta0 = (2.0 * w0 * xi / opt->m) * (2.0 * w0 * xi / opt->m);
+    opt->h = h0;
+    opt->c = 2.0 * xi * xi / eta0;
+    opt->w = w0;
+    opt->eta = eta0;
+    opt->xi = xi;
+}
+
+void k_vec(int M, double* box, double* k1, double* k2, double* k3)
+{
+    const int pos = (M % 2 == 0) ? M / 2 : (M - 1) / 2 + 1;
+    for (int i = 0; i < M; ++i) {
+        const double f = (double)(i < pos ? i : i - M);
+        k1[i] = 2.0 * PI / box[0] * f;
+        k2[i] = 2.0 * PI / box[1] * f;
+        k3[i] = 2.0 * PI / box[2] * f;
+    }
+}
+
+void scaling(double scalar, double* Z, int n1, int n2, int n3, double* box)
+{
+    run("scaling", [&] {
+        SE_REQUIRE(n1 == n2 && n2 == n3, SE_ERR_ARG, "scaling: the reference indexes a cubic grid only");
+        const size_t n = (size_t)n1 * n1 * n1;
+        double* d = dev<double>(20, n);
+        launch_scaling(scalar, d, n1, box[0], stream());
+        down(Z, d, n);
+        finish();
+    });
+}
+
+void product_rc(t_complex* a, double* b, t_complex* c, int n1, int n2, int n3, int flag)
+{
+    run("product_rc", [&] {
+        if (flag != 1 && flag != -1) return;   // the reference's switch does nothing else
+        const size_t n = (size_t)n1 * n2 * n3;
+        double* da = up(21, (double*)a, 2 * n);
+        double* db = up(22, b, n);
+        double* dc = dev<double>(23, 2 * n);
+        launch_product_rc(da, db, dc, (long)n, flag, stream());
+        down((double*)c, dc, 2 * n);
+        finish();
+    });
+}
+
+void SE_FGG_FCN_params(SE_FGG_params* params, const SE_opt* opt, int N)
+{
+    params->N = N;
+    params->P = opt->P;
+    params->P_half = fgg_half(opt->P);
+    params->c = opt->c;
+    params->d = std::pow(params->c / PI, 1.5);
+    params->h = opt->box[0] / opt->M;
+    params->a = -FGG_INF;
+    for (int d = 0; d < 3; ++d) {
+        params->dims[d] = opt->M;
+        params->npdims[d] = opt->M + params->P;
+    }
+}
+
+void SE_fgg_int_force(double* x, double* q, double* H, int N, SE_opt opt, double* force_out)
+{
+    SE_FGG_params params;
+    SE_FGG_FCN_params(&params, &opt, N);
+    SE_FGG_work work;
+    SE_FGG_allocate_workspace(&work, &params, 1, 0);
+    SE_state st = {x, q, nullptr};
+    SE_FGG_base_gaussian(&work, &params);
+    SE_FGG_extend_fcn(&work, H, &params);
+    SE_FGG_int_force(force_out, &work, &st, &params);
+    SE_FGG_free_workspace(&work);
+}
+
+void copy_r2c(double* in, t_complex* out, int n)
+{
+    for (int i = 0; i < n; i++) {
+        out[i][0] = in[i];
+        out[i][1] = 0.;
+    }
+}
+
+void copy_c2r(t_complex* in, double* out, int n)
+{
+    for (int i = 0; i < n; i++) out[i] = in[i][0];
+}
+
+double SE_init_params(int* M, int* P, double* m, double eps, double L, double xi, const double* q, int N)
+{
+    (void)eps;
+    double Q = 0.;
+    for (int i = 0; i < N; i++) Q += q[i] * q[i];
+    const double c = .95;
+    const int iP = *P;
+    const double m_l = c * std::sqrt(PI * iP);
+    const double k_inf = 2. * xi * L;
+    int M_l = (int)std::ceil(2. * k_inf + 1) + 2;
+    if (M_l < iP) M_l = iP;
+    // the reference's AVX build rounds M up to a multiple of 4 (SE_general.c:559-563)
+    if (M_l % 4) M_l += 4 - M_l % 4;
+    *m = m_l;
+    *M = M_l;
+    return Q;
+}
+
+int spectral_ewald(double* x, double* q, SE_opt opt, double xi, double* force_phi, double* phi_energy)
+{
+    return run("spectral_ewald", [&] {
+        parse_params(&opt, xi);
+        const int N = opt.N;
+        const int quant = g_quantity;
+        se_plan* pl = driver_plan(opt, xi, N, g_precision);
+        double* dx = up(30, x, 3 * (size_t)N);
+        double* dq = up(31, q, (size_t)N);
+        double* dphi = nullptr;
+        double* dF = nullptr;
+        if (quant & SE_POTENTIAL) dphi = dev<double>(32, (size_t)std::max(N, 1));
+        if (quant & SE_FORCE) dF = dev<double>(33, 3 * (size_t)std::max(N, 1));
+        int rc = se_solve(pl, N, dx, dq, dphi, dF);
+        if (rc != SE_OK) throw se::Error{rc, se_gpu_last_error()};
+        if (quant == SE_POTENTIAL) {
+            down(force_phi, dphi, (size_t)N);
+        } else {
+            down(force_phi, dF, 3 * (size_t)N);
+            if ((quant & SE_POTENTIAL) && phi_energy) down(phi_energy, dphi, (size_t)N);
+        }
+        finish();
+    });
+}
+
+}  // extern "C"

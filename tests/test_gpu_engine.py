@@ -1,0 +1,170 @@
+"""GPU parity of the device-resident SE3P engine (include/se_gpu.h) against the
+CPU oracle: FP64 to <= 1e-12 rms-relative, FP32 to the stated FP32 tolerance, plus
+size-independent properties (periodic images, adjointness, tiling invariance) and
+edge cases (empty, single particle, boundary positions, heavy tiles)."""
+import numpy as np
+import pytest
+import torch
+
+import se_fast_gridding_amd as se
+from conftest import rms_rel
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+FP32_TOL = 1e-5     # rms-relative, FP32 spread/FFT/gather vs FP64 oracle
+
+
+def system(N, seed, clustered=False):
+    rng = np.random.default_rng(seed)
+    if clustered:
+        c = rng.random((3, 4))
+        lab = rng.integers(0, 4, N)
+        x = (c[:, lab] + 0.01 * rng.standard_normal((3, N))) % 1.0
+    else:
+        x = rng.random((3, N))
+    q = rng.uniform(-1, 1, N)
+    q -= q.mean()
+    return np.ascontiguousarray(x.reshape(-1)), q
+
+
+def solve(x, q, M, P, xi, prec, force=True, L=1.0, **kw):
+    N = len(q)
+    plan = se.Plan(M, P, L, xi, max(N, 1), precision=prec, **kw)
+    xt = torch.tensor(x, device=DEV)
+    qt = torch.tensor(q, device=DEV)
+    phi = torch.zeros(N, dtype=torch.float64, device=DEV)
+    F = torch.zeros(3 * N, dtype=torch.float64, device=DEV) if force else None
+    plan.solve(xt, qt, phi, F)
+    torch.cuda.synchronize()
+    info = plan.info()
+    plan.close()
+    return phi.cpu().numpy(), (F.cpu().numpy() if force else None), info
+
+
+CASES = [  # N, M, P, xi, L
+    (100, 32, 21, 6.0, 1.0),
+    (300, 16, 8, 4.0, 1.0),
+    (200, 20, 11, 5.0, 1.0),
+    (150, 12, 16, 3.0, 1.0),
+    (250, 24, 5, 6.0, 2.5),
+    (120, 18, 12, 4.5, 1.0),
+    (64, 10, 3, 2.0, 1.0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d_M%d_P%d" % c[:3])
+def test_fp64_solve_matches_oracle(oracle, case):
+    N, M, P, xi, L = case
+    print("case fp64", case, flush=True)
+    x, q = system(N, N + M + P)
+    x = x * L
+    m = 0.95 * np.sqrt(np.pi * P)
+    phi, F, _ = solve(x, q, M, P, xi, se.PREC_FP64, L=L)
+    assert rms_rel(phi, oracle.se3p(x, q, L, M, P, m, xi)) < 1e-12
+    assert rms_rel(F, oracle.se3p(x, q, L, M, P, m, xi, force=True)) < 1e-12
+
+
+@pytest.mark.parametrize("case", CASES[:4], ids=lambda c: "N%d_M%d_P%d" % c[:3])
+def test_fp32_solve_matches_oracle(oracle, case):
+    N, M, P, xi, L = case
+    print("case fp32", case, flush=True)
+    x, q = system(N, 7 * N + P)
+    m = 0.95 * np.sqrt(np.pi * P)
+    phi, F, _ = solve(x, q, M, P, xi, se.PREC_FP32)
+    e_phi = rms_rel(phi, oracle.se3p(x, q, 1.0, M, P, m, xi))
+    e_F = rms_rel(F, oracle.se3p(x, q, 1.0, M, P, m, xi, force=True))
+    print("fp32 rms-rel phi %.2e F %.2e" % (e_phi, e_F))
+    assert e_phi < FP32_TOL and e_F < FP32_TOL
+
+
+def test_spread_grid_matches_oracle_wrap(oracle):
+    """Spread + fused wrap equals the reference's column-major H_per (transposed)."""
+    N, M, P = 180, 14, 9
+    x, q = system(N, 3)
+    c = 50.0
+    for prec, tol in ((se.PREC_FP64, 1e-13), (se.PREC_FP32, 2e-6)):
+        plan = se.Plan(M, P, 1.0, 3.0, N, precision=prec)
+        plan.spread(torch.tensor(x, device=DEV), torch.tensor(q, device=DEV))
+        torch.cuda.synchronize()
+        G = plan.grid().double().cpu().numpy()
+        p = oracle.pack_params(N, M, P, plan_c(M, P, 3.0), 1.0 / M)
+        zs = oracle.base_gaussian(p)
+        Hper = oracle.wrap(3, p, oracle.grid_split(p, q, oracle.expand_all(3, x, p), zs))
+        ref = Hper.reshape(M, M, M, order="F")
+        plan.close()
+        assert np.max(np.abs(G - ref)) <= tol * np.max(np.abs(ref))
+
+
+def plan_c(M, P, xi, L=1.0):
+    m = 0.95 * np.sqrt(np.pi * P)
+    h = L / M
+    w = P * h / 2
+    eta = (2 * w * xi / m) ** 2
+    return 2 * xi * xi / eta
+
+
+def test_adjointness_spread_interp():
+    """<spread(q), G> == h^-3 sum_n q_n interp(G)(x_n) for a random grid G (FP64)."""
+    N, M, P = 300, 16, 10
+    x, q = system(N, 5)
+    plan = se.Plan(M, P, 1.0, 4.0, N, precision=se.PREC_FP64)
+    xt, qt = torch.tensor(x, device=DEV), torch.tensor(q, device=DEV)
+    plan.spread(xt, qt)
+    A = plan.grid()
+    G = torch.tensor(np.random.default_rng(9).standard_normal((M, M, M)), device=DEV)
+    plan.set_grid(G)
+    phi = torch.zeros(N, dtype=torch.float64, device=DEV)
+    plan.interp(phi, None)
+    torch.cuda.synchronize()
+    lhs = float((A * G).sum())
+    rhs = float((qt * phi).sum()) / (1.0 / M) ** 3
+    plan.close()
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+
+
+def test_periodic_images_and_boundaries(oracle):
+    """Positions shifted by whole boxes give the same result; x=0 and x just below L work."""
+    N, M, P, xi = 90, 16, 8, 4.0
+    x, q = system(N, 11)
+    x[:5] = 0.0
+    x[5:10] = np.nextafter(1.0, 0.0)
+    phi0, F0, _ = solve(x, q, M, P, xi, se.PREC_FP64)
+    m = 0.95 * np.sqrt(np.pi * P)
+    assert rms_rel(phi0, oracle.se3p(x, q, 1.0, M, P, m, xi)) < 1e-12
+    shift = np.random.default_rng(2).integers(-3, 4, x.shape).astype(float)
+    phi1, F1, _ = solve(x + shift, q, M, P, xi, se.PREC_FP64)
+    assert rms_rel(phi1, phi0) < 1e-11 and rms_rel(F1, F0) < 1e-11
+
+
+def test_empty_and_single_particle(oracle):
+    phi, F, info = solve(np.zeros(0), np.zeros(0), 16, 8, 4.0, se.PREC_FP64)
+    assert phi.size == 0 and info["items"] == 0
+    x = np.array([0.3, 0.6, 0.9])
+    q = np.array([1.0])
+    m = 0.95 * np.sqrt(np.pi * 8)
+    phi, F, _ = solve(x, q, 16, 8, 4.0, se.PREC_FP64)
+    assert rms_rel(phi, oracle.se3p(x, q, 1.0, 16, 8, m, 4.0)) < 1e-12
+    assert np.max(np.abs(F)) < 1e-10   # a lone charge feels no k-space force
+
+
+@pytest.mark.parametrize("prec", [se.PREC_FP64, se.PREC_FP32])
+def test_tiling_and_chunking_invariance(prec):
+    """Heavy tiles split into many work items give the same answer as one item per tile."""
+    N, M, P, xi = 2000, 24, 8, 5.0
+    x, q = system(N, 13, clustered=True)
+    ref, Fref, info0 = solve(x, q, M, P, xi, prec)
+    tol = 1e-12 if prec == se.PREC_FP64 else 2e-6
+    for kw in (dict(chunk=16), dict(tile=4, chunk=64, block=128), dict(tile=3, block=64), dict(tile=12)):
+        phi, F, info = solve(x, q, M, P, xi, prec, **kw)
+        assert rms_rel(phi, ref) < tol and rms_rel(F, Fref) < tol, kw
+    assert solve(x, q, M, P, xi, prec, chunk=16)[2]["items"] > info0["items"]
+
+
+def test_capacity_is_enforced():
+    plan = se.Plan(16, 8, 1.0, 4.0, 10, precision=se.PREC_FP64)
+    x = torch.zeros(3 * 11, dtype=torch.float64, device=DEV)
+    q = torch.zeros(11, dtype=torch.float64, device=DEV)
+    with pytest.raises(RuntimeError, match="capacity"):
+        plan.solve(x, q, torch.zeros(11, dtype=torch.float64, device=DEV))
+    plan.close()
