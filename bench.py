@@ -109,7 +109,8 @@ def run(args):
     F = torch.empty(3 * N, dtype=torch.float64, device=dev)
     prec = se.PREC_FP32 if args.precision == "fp32" else se.PREC_FP64
     stream = torch.cuda.current_stream()
-    plan = se.Plan(M, P, 1.0, xi, N, precision=prec, stream=stream)
+    plan = se.Plan(M, P, 1.0, xi, N, precision=prec, stream=stream, tile=args.tile, chunk=args.chunk,
+                   block=args.block, generic=args.generic)
 
     for _ in range(args.warmup):
         plan.solve(xt, qt, phi, F)
@@ -140,10 +141,11 @@ def run(args):
     peak = FP32_PEAK_TFLOPS if prec == se.PREC_FP32 else FP64_PEAK_TFLOPS
     spread_flops = 2.0 * N * P ** 3
     gather_flops = 4.0 * N * P ** 3
+    kind = "pencil" if info["pencil"] else "tiles"
     if ms["spread_kernel"] >= ms["gather_kernel"]:
-        kname, kflops, kms = "k_spread_tiles", spread_flops, ms["spread_kernel"]
+        kname, kflops, kms = "k_spread_" + kind, spread_flops, ms["spread_kernel"]
     else:
-        kname, kflops, kms = "k_interp_tiles", gather_flops, ms["gather_kernel"]
+        kname, kflops, kms = "k_interp_" + kind, gather_flops, ms["gather_kernel"]
     achieved = kflops / (kms * 1e-3) / 1e12
 
     # accuracy: FP32 P=16 vs FP64 P=21 on the same system (FGG floor ~1e-13 at P=21)
@@ -207,6 +209,10 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-check", dest="check", action="store_false")
+    ap.add_argument("--tile", type=int, default=0, help="tile side (0 = plan default)")
+    ap.add_argument("--chunk", type=int, default=0, help="max particles per work item (0 = default)")
+    ap.add_argument("--block", type=int, default=0, help="threads per work-group (0 = default)")
+    ap.add_argument("--generic", action="store_true", help="one-particle-per-wave kernels instead of pencils")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has exactly one; "

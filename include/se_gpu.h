@@ -50,7 +50,12 @@ typedef struct {
     int tile;          /* tile side in grid cells, 0 = automatic */
     int chunk;         /* max particles per work item, 0 = automatic */
     int block;         /* threads per work-group for spread/gather, 0 = automatic */
+    int flags;         /* SE_PLAN_* bits */
 } se_plan_desc;
+
+/* Use the generic one-particle-per-wave kernels even where the FP32 register-ring
+ * (pencil) kernels exist (for A/B comparisons and tests). */
+#define SE_PLAN_GENERIC_KERNELS 1
 
 int se_plan_create(se_plan** plan, const se_plan_desc* desc, void* hip_stream);
 int se_plan_destroy(se_plan* plan);
@@ -83,7 +88,9 @@ int se_plan_enable_timing(se_plan* plan, int on);
 int se_plan_timings(se_plan* plan, float ms[8]);
 
 /* Tile/work-item geometry chosen for the plan: out[0]=tile, [1]=chunk, [2]=block,
- * [3]=number of tiles, [4]=work items of the last spread (syncs). */
+ * [3]=number of tiles, [4]=work items of the last spread (syncs), [5]=particles of the
+ * last spread discarded as non-finite, [6]=sub-grid side S, [7]=1 if the pencil
+ * kernels run. */
 int se_plan_info(se_plan* plan, int out[8]);
 
 /* Energy E = 1/2 sum phi q (FP64, on the device), into *d_energy (device double). */

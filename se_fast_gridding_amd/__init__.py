@@ -60,7 +60,9 @@ class se_plan_desc(C.Structure):
     _fields_ = [("periodicity", C.c_int), ("M", C.c_int), ("P", C.c_int),
                 ("box", C.c_double), ("xi", C.c_double), ("m", C.c_double),
                 ("precision", C.c_int), ("n_max", C.c_int), ("tile", C.c_int),
-                ("chunk", C.c_int), ("block", C.c_int)]
+                ("chunk", C.c_int), ("block", C.c_int), ("flags", C.c_int)]
+
+PLAN_GENERIC_KERNELS = 1
 
 
 _D = C.POINTER(C.c_double)
@@ -204,10 +206,11 @@ class Plan:
     force SoA (3N,) in the reference's (q/2) grad(phi) convention (SE_fgg.c:2291-2307)."""
 
     def __init__(self, M, P, box, xi, n_max, precision=PREC_FP64, m=0.0, tile=0, chunk=0, block=0,
-                 stream=None):
+                 stream=None, generic=False):
         import torch
         self._torch = torch
-        d = se_plan_desc(3, M, P, float(box), float(xi), float(m), precision, int(n_max), tile, chunk, block)
+        d = se_plan_desc(3, M, P, float(box), float(xi), float(m), precision, int(n_max), tile, chunk, block,
+                         PLAN_GENERIC_KERNELS if generic else 0)
         self.desc = d
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         h = C.c_void_p()
@@ -272,7 +275,7 @@ class Plan:
     def info(self):
         out = (C.c_int * 8)()
         check(capi().se_plan_info(self.h, out), "se_plan_info")
-        keys = ["tile", "chunk", "block", "ntiles", "items", "discarded", "S", "max_items"]
+        keys = ["tile", "chunk", "block", "ntiles", "items", "discarded", "S", "pencil"]
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def nbytes(self):

@@ -52,35 +52,41 @@ struct DevMem {
     size_t bytes = 0;
 };
 
-// Tiling rule: largest tile side (<= 16) whose S^3 LDS sub-grid fits the budget;
-// FP32 targets two work-groups per CU, FP64 one.
-static Tiling choose_tiling(int esize, int P, int M, int n_cap, int tile, int chunk, int block, int lds_max)
+// Tiling rule.  Spread sub-grids accumulate in FP64 LDS (S^3 * 8 bytes), gather
+// sub-grids hold the grid precision.  Default tile side: the largest (<= 16; even
+// and <= 12 for the pencil kernels) whose FP64 sub-grid fits in LDS.
+static Tiling choose_tiling(int esize, int P, int M, int n_cap, int tile, int chunk, int block, int lds_max,
+                            bool allow_pencil)
 {
     Tiling tl{};
-    const int budget = std::min(lds_max, esize == 4 ? 64 * 1024 : 160 * 1024);
+    tl.pencil = (allow_pencil && esize == 4 && pencil_supported(P)) ? 1 : 0;
+    const int tmax = tl.pencil ? 12 : 16;
     int TS = tile;
     if (TS <= 0) {
         TS = 0;
-        for (int t = 16; t >= 1; --t) {
+        for (int t = tmax; t >= 1; t -= (tl.pencil ? 2 : 1)) {
             long S = t + P - 1;
-            if (S * S * S * esize <= budget) { TS = t; break; }
+            if (S * S * S * 8 <= lds_max) { TS = t; break; }
         }
-        if (TS == 0) {
-            for (int t = 16; t >= 1; --t) {
-                long S = t + P - 1;
-                if (S * S * S * esize <= lds_max) { TS = t; break; }
-            }
-        }
-        SE_REQUIRE(TS > 0, SE_ERR_ARG, "support P too large for the LDS sub-grid at this precision");
+        SE_REQUIRE(TS > 0, SE_ERR_ARG, "support P too large for the LDS sub-grid");
     }
     TS = std::min(TS, M);
+    if (tl.pencil && (TS & 1)) TS -= 1;   // pencils pair columns: even tile side
+    if (tl.pencil && TS < 2) {
+        tl.pencil = 0;
+        TS = std::max(1, std::min(tile > 0 ? tile : M, M));
+    }
     SE_REQUIRE(TS >= 1 && TS <= 255, SE_ERR_ARG, "tile side out of range");
     tl.TS = TS;
     tl.NT = (M + TS - 1) / TS;
     tl.S = TS + P - 1;
     tl.Szp = tl.S;
-    SE_REQUIRE((long)tl.S * tl.S * tl.Szp * esize <= lds_max, SE_ERR_ARG, "tile sub-grid exceeds LDS");
+    SE_REQUIRE((long)tl.S * tl.S * tl.Szp * 8 <= lds_max, SE_ERR_ARG, "tile sub-grid exceeds LDS");
     tl.ntiles = tl.NT * tl.NT * tl.NT;
+    tl.PX = (TS + 1) / 2;
+    tl.K = tl.PX * tl.PX * TS;
+    SE_REQUIRE((long)tl.ntiles * tl.K < (1l << 30), SE_ERR_ARG, "grid too large for the sort keys");
+    tl.nbins = tl.ntiles * tl.K;
     tl.chunk = chunk > 0 ? chunk : 2048;
     tl.block = block > 0 ? block : 512;
     SE_REQUIRE(tl.block % 64 == 0 && tl.block <= 512, SE_ERR_ARG, "block must be a multiple of 64, <= 512");
@@ -104,9 +110,9 @@ struct se_plan {
     int n_cap = 0;
     int last_N = -1;
     // device buffers
-    int *key = nullptr, *rank = nullptr, *count = nullptr, *tile_start = nullptr, *nchunk = nullptr,
-        *chunk_start = nullptr, *item_tile = nullptr, *item_beg = nullptr, *item_end = nullptr,
-        *n_items = nullptr, *ncover = nullptr;
+    int *key = nullptr, *rank = nullptr, *count = nullptr, *row_start = nullptr, *tile_count = nullptr,
+        *nchunk = nullptr, *chunk_start = nullptr, *item_tile = nullptr, *item_beg = nullptr,
+        *item_end = nullptr, *n_items = nullptr, *ncover = nullptr;
     int2* cover = nullptr;
     void *rec = nullptr, *partials = nullptr, *grid = nullptr, *spec = nullptr, *cub_tmp = nullptr,
          *fft_work = nullptr;
@@ -210,11 +216,13 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max)
     double eta, m;
     derive(d, g, eta, m);
     const int es = d->precision == SE_PREC_FP32 ? 4 : 8;
-    Tiling tl = choose_tiling(es, d->P, d->M, d->n_max, d->tile, d->chunk, d->block, lds_max);
+    Tiling tl = choose_tiling(es, d->P, d->M, d->n_max, d->tile, d->chunk, d->block, lds_max,
+                              !(d->flags & SE_PLAN_GENERIC_KERNELS));
     const size_t M = d->M, N = d->n_max;
     size_t b = 0;
     b += 2 * N * 4;                                   // key, rank
-    b += 5 * (size_t)(tl.ntiles + 1) * 4;             // count, tile_start, nchunk, chunk_start
+    b += 2 * (size_t)(tl.nbins + 1) * 4;              // count, row_start
+    b += 4 * (size_t)(tl.ntiles + 2) * 4;             // tile_count, nchunk, chunk_start
     b += 3 * (size_t)tl.max_items * 4;                // items
     b += N * (es == 4 ? 32 : 48);                     // records
     b += (size_t)tl.max_items * tl.part_elems * es;   // partials
@@ -239,19 +247,26 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q)
     hipStream_t s = pl->stream;
     const Tiling& tl = pl->tl;
     pl->rec_ev(0);
-    launch_fill_i32(pl->count, tl.ntiles + 1, 0, s);
+    launch_fill_i32(pl->count, (long)tl.nbins + 1, 0, s);
     launch_bin_keys<T>(x, N, pl->g, tl, pl->key, pl->rank, pl->count, s);
-    launch_nchunk(pl->count, tl.ntiles, tl.chunk, pl->nchunk, s);
     size_t tb = pl->cub_bytes;
-    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->count, pl->tile_start, tl.ntiles + 1, s));
+    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->count, pl->row_start, tl.nbins + 1, s));
+    launch_tile_chunks(pl->row_start, tl.ntiles, tl.K, tl.chunk, pl->tile_count, pl->nchunk, s);
     tb = pl->cub_bytes;
     SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->nchunk, pl->chunk_start, tl.ntiles + 1, s));
-    launch_items(pl->count, pl->tile_start, pl->chunk_start, tl.ntiles, tl.chunk, pl->item_tile, pl->item_beg,
-                 pl->item_end, pl->n_items, s);
-    launch_scatter<T>(x, q, N, pl->g, tl, pl->key, pl->rank, pl->tile_start, (Rec<T>*)pl->rec, s);
+    launch_items(pl->tile_count, pl->row_start, pl->chunk_start, tl.ntiles, tl.K, tl.chunk, pl->item_tile,
+                 pl->item_beg, pl->item_end, pl->n_items, s);
+    launch_scatter<T>(x, q, N, pl->g, tl, pl->key, pl->rank, pl->row_start, (Rec<T>*)pl->rec, s);
     pl->rec_ev(1);
-    launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items, pl->g,
-                           tl, (T*)pl->partials, s);
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+        if (tl.pencil)
+            done = launch_spread_pencil((const Rec<float>*)pl->rec, pl->item_beg, pl->item_end, pl->n_items, pl->g,
+                                        tl, (float*)pl->partials, s);
+    }
+    if (!done)
+        launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
+                               pl->g, tl, (T*)pl->partials, s);
     pl->rec_ev(2);
     launch_merge<T>((const T*)pl->partials, pl->chunk_start, pl->cover, pl->ncover, pl->g, tl, (T*)pl->grid, s);
     pl->rec_ev(3);
@@ -262,8 +277,16 @@ template <typename T>
 void run_interp(se_plan* pl, double* phi, double* force)
 {
     pl->rec_ev(4);
-    launch_interp_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
-                           (const T*)pl->grid, pl->g, pl->tl, pl->last_N, phi, force, pl->stream);
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+        if (pl->tl.pencil)
+            done = launch_interp_pencil((const Rec<float>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end,
+                                        pl->n_items, (const float*)pl->grid, pl->g, pl->tl, pl->last_N, phi, force,
+                                        pl->stream);
+    }
+    if (!done)
+        launch_interp_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
+                               (const T*)pl->grid, pl->g, pl->tl, pl->last_N, phi, force, pl->stream);
     pl->rec_ev(5);
     SE_HIP(hipGetLastError());
 }
@@ -314,13 +337,15 @@ int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
         pl->stream = (hipStream_t)stream;
         pl->n_cap = std::max(d->n_max, 1);
         const int lds_max = lds_limit();
-        pl->tl = choose_tiling(pl->esize, d->P, d->M, pl->n_cap, d->tile, d->chunk, d->block, lds_max);
+        pl->tl = choose_tiling(pl->esize, d->P, d->M, pl->n_cap, d->tile, d->chunk, d->block, lds_max,
+                               !(d->flags & SE_PLAN_GENERIC_KERNELS));
         const Tiling& tl = pl->tl;
         const size_t M = d->M, N = pl->n_cap;
         pl->key = (int*)pl->alloc(N * 4);
         pl->rank = (int*)pl->alloc(N * 4);
-        pl->count = (int*)pl->alloc((tl.ntiles + 1) * 4);
-        pl->tile_start = (int*)pl->alloc((tl.ntiles + 1) * 4);
+        pl->count = (int*)pl->alloc(((size_t)tl.nbins + 1) * 4);
+        pl->row_start = (int*)pl->alloc(((size_t)tl.nbins + 1) * 4);
+        pl->tile_count = (int*)pl->alloc((tl.ntiles + 1) * 4);
         pl->nchunk = (int*)pl->alloc((tl.ntiles + 1) * 4);
         pl->chunk_start = (int*)pl->alloc((tl.ntiles + 2) * 4);
         pl->item_tile = (int*)pl->alloc((size_t)tl.max_items * 4);
@@ -332,7 +357,7 @@ int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
         pl->grid = pl->alloc(M * M * M * pl->esize);
         pl->spec = pl->alloc(M * M * (M / 2 + 1) * 2 * pl->esize);
         pl->gtab = (double*)pl->alloc((2 * M + 1) * sizeof(double));
-        SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->count, pl->tile_start, tl.ntiles + 1,
+        SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->count, pl->row_start, tl.nbins + 1,
                                                 pl->stream));
         pl->cub_tmp = pl->alloc(pl->cub_bytes);
         build_cover(pl);
@@ -477,11 +502,11 @@ int se_plan_info(se_plan* pl, int out[8])
         SE_HIP(hipStreamSynchronize(pl->stream));
         int ni = 0, bad = 0;
         SE_HIP(hipMemcpy(&ni, pl->n_items, sizeof(int), hipMemcpyDeviceToHost));
-        SE_HIP(hipMemcpy(&bad, pl->count + pl->tl.ntiles, sizeof(int), hipMemcpyDeviceToHost));
+        SE_HIP(hipMemcpy(&bad, pl->count + pl->tl.nbins, sizeof(int), hipMemcpyDeviceToHost));
         out[4] = ni;
         out[5] = bad;
         out[6] = pl->tl.S;
-        out[7] = pl->tl.max_items;
+        out[7] = pl->tl.pencil;
     });
 }
 

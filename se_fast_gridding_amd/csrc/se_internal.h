@@ -51,12 +51,16 @@ struct Tiling {
     int TS, NT, S;    // tile side, tiles per side (ceil(M/TS)), sub-grid side
     int Szp;          // z pitch of the LDS sub-grid (>= S)
     int ntiles;       // NT^3
+    int PX;           // 2x2-column pencils per tile side: ceil(TS/2)
+    int K;            // sort keys per tile: PX*PX*TS
+    int nbins;        // ntiles*K (the discard bucket is nbins)
     int chunk;        // max particles per work item
     int block;        // threads per work-group
     int sub_elems;    // S*S*Szp
     int part_elems;   // S*S*S
     int max_items;    // ntiles + n_cap / chunk + 1
     int ncov;         // max covering (tile, offset) pairs per coordinate
+    int pencil;       // 1: register-ring pencil kernels (FP32, P in the compiled set)
 };
 
 // Particle record written by the bucketing scatter, read by spread and gather.
@@ -82,14 +86,24 @@ void launch_fill_bytes0(void* p, size_t bytes, hipStream_t s);
 template <typename T>
 void launch_bin_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key,
                      int* rank, int* count, hipStream_t s);
-void launch_nchunk(const int* count, int ntiles, int chunk, int* nchunk, hipStream_t s);
-void launch_items(const int* count, const int* tile_start, const int* chunk_start, int ntiles,
-                  int chunk, int* item_tile, int* item_beg, int* item_end, int* n_items,
+void launch_tile_chunks(const int* row_start, int ntiles, int K, int chunk, int* tile_count,
+                        int* nchunk, hipStream_t s);
+void launch_items(const int* tile_count, const int* row_start, const int* chunk_start, int ntiles,
+                  int K, int chunk, int* item_tile, int* item_beg, int* item_end, int* n_items,
                   hipStream_t s);
 template <typename T>
 void launch_scatter(const double* x, const double* q, int N, const Geom& g, const Tiling& tl,
-                    const int* key, const int* rank, const int* tile_start, Rec<T>* rec,
+                    const int* key, const int* rank, const int* row_start, Rec<T>* rec,
                     hipStream_t s);
+// pencil kernels (se_pencil.hip): FP32 only; return false if P has no compiled pencil kernel
+bool pencil_supported(int P);
+bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int* item_end,
+                          const int* n_items, const Geom& g, const Tiling& tl, float* partials,
+                          hipStream_t s);
+bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg,
+                          const int* item_end, const int* n_items, const float* grid,
+                          const Geom& g, const Tiling& tl, int N, double* phi, double* force,
+                          hipStream_t s);
 template <typename T>
 void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const Geom& g,

@@ -91,7 +91,7 @@ void launch_fill_bytes0(void* p, size_t bytes, hipStream_t s)
 }
 
 // ================================================================================
-// Bucketing: particle -> tile of its window start (counting sort)
+// Bucketing: particle -> (tile, pencil, row) of its window start (counting sort)
 // ================================================================================
 __device__ __forceinline__ int window_start(double x, const Geom& g, double& t, bool& ok)
 {
@@ -101,6 +101,19 @@ __device__ __forceinline__ int window_start(double x, const Geom& g, double& t, 
     int iota = g.odd ? (int)round(u) : (int)floor(u);
     t = x - g.h * iota;
     return imod(iota + g.first, g.M);
+}
+
+// Counting-sort key of a particle: its window start cell, ordered tile-major, then
+// by 2x2-column pencil (px, pz), then by row y inside the tile:
+//   key = tile * K + ((px * PX + pz) * TS + ly),  K = PX * PX * TS,  PX = ceil(TS/2).
+// Sorting by this key groups particles by tile (for the work items) and, inside a
+// tile, into pencils walked row by row (for the pencil kernels).
+__device__ __forceinline__ int row_key(int w0, int w1, int w2, const Tiling& tl)
+{
+    const int tx = w0 / tl.TS, ty = w1 / tl.TS, tz = w2 / tl.TS;
+    const int lx = w0 - tx * tl.TS, ly = w1 - ty * tl.TS, lz = w2 - tz * tl.TS;
+    const int tile = (tx * tl.NT + ty) * tl.NT + tz;
+    return tile * tl.K + (((lx >> 1) * tl.PX + (lz >> 1)) * tl.TS + ly);
 }
 
 template <typename T>
@@ -114,9 +127,8 @@ __global__ void k_bin_keys(const double* __restrict__ x, int N, Geom g, Tiling t
     int w0 = window_start(x[n], g, t, ok0);
     int w1 = window_start(x[n + (long)N], g, t, ok1);
     int w2 = window_start(x[n + 2l * N], g, t, ok2);
-    int k = (ok0 && ok1 && ok2)
-                ? ((w0 / tl.TS) * tl.NT + (w1 / tl.TS)) * tl.NT + (w2 / tl.TS)
-                : tl.ntiles;   // non-finite or absurd positions go to a discard bucket
+    // non-finite or absurd positions go to a discard bucket after all tiles
+    int k = (ok0 && ok1 && ok2) ? row_key(w0, w1, w2, tl) : tl.nbins;
     key[n] = k;
     rank[n] = atomicAdd(&count[k], 1);
 }
@@ -129,27 +141,37 @@ void launch_bin_keys(const double* x, int N, const Geom& g, const Tiling& tl, in
     hipLaunchKernelGGL(k_bin_keys<T>, dim3(cdiv(N, 256)), dim3(256), 0, s, x, N, g, tl, key, rank, count);
 }
 
-__global__ void k_nchunk(const int* __restrict__ count, int ntiles, int chunk, int* __restrict__ nchunk)
+// Per tile: particle count (from the row-key starts) and number of work items.
+__global__ void k_tile_chunks(const int* __restrict__ row_start, int ntiles, int K, int chunk,
+                              int* __restrict__ tile_count, int* __restrict__ nchunk)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t > ntiles) return;
-    nchunk[t] = (t < ntiles) ? (count[t] + chunk - 1) / chunk : 0;
+    if (t == ntiles) {
+        nchunk[t] = 0;
+        return;
+    }
+    const int c = row_start[(long)(t + 1) * K] - row_start[(long)t * K];
+    tile_count[t] = c;
+    nchunk[t] = (c + chunk - 1) / chunk;
 }
 
-void launch_nchunk(const int* count, int ntiles, int chunk, int* nchunk, hipStream_t s)
+void launch_tile_chunks(const int* row_start, int ntiles, int K, int chunk, int* tile_count, int* nchunk,
+                        hipStream_t s)
 {
-    hipLaunchKernelGGL(k_nchunk, dim3(cdiv(ntiles + 1, 256)), dim3(256), 0, s, count, ntiles, chunk, nchunk);
+    hipLaunchKernelGGL(k_tile_chunks, dim3(cdiv(ntiles + 1, 256)), dim3(256), 0, s, row_start, ntiles, K, chunk,
+                       tile_count, nchunk);
 }
 
-__global__ void k_items(const int* __restrict__ count, const int* __restrict__ tile_start,
-                        const int* __restrict__ chunk_start, int ntiles, int chunk,
+__global__ void k_items(const int* __restrict__ tile_count, const int* __restrict__ row_start,
+                        const int* __restrict__ chunk_start, int ntiles, int K, int chunk,
                         int* __restrict__ item_tile, int* __restrict__ item_beg,
                         int* __restrict__ item_end, int* __restrict__ n_items)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) *n_items = chunk_start[ntiles];
     if (t >= ntiles) return;
-    const int b = tile_start[t], e = b + count[t];
+    const int b = row_start[(long)t * K], e = b + tile_count[t];
     int it = chunk_start[t];
     for (int p = b; p < e; p += chunk, ++it) {
         item_tile[it] = t;
@@ -158,17 +180,17 @@ __global__ void k_items(const int* __restrict__ count, const int* __restrict__ t
     }
 }
 
-void launch_items(const int* count, const int* tile_start, const int* chunk_start, int ntiles, int chunk,
-                  int* item_tile, int* item_beg, int* item_end, int* n_items, hipStream_t s)
+void launch_items(const int* tile_count, const int* row_start, const int* chunk_start, int ntiles, int K,
+                  int chunk, int* item_tile, int* item_beg, int* item_end, int* n_items, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_items, dim3(cdiv(ntiles, 256)), dim3(256), 0, s, count, tile_start, chunk_start,
-                       ntiles, chunk, item_tile, item_beg, item_end, n_items);
+    hipLaunchKernelGGL(k_items, dim3(cdiv(ntiles, 256)), dim3(256), 0, s, tile_count, row_start, chunk_start,
+                       ntiles, K, chunk, item_tile, item_beg, item_end, n_items);
 }
 
 template <typename T>
 __global__ void k_scatter(const double* __restrict__ x, const double* __restrict__ q, int N, Geom g,
                           Tiling tl, const int* __restrict__ key, const int* __restrict__ rank,
-                          const int* __restrict__ tile_start, Rec<T>* __restrict__ rec)
+                          const int* __restrict__ row_start, Rec<T>* __restrict__ rec)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -185,25 +207,27 @@ __global__ void k_scatter(const double* __restrict__ x, const double* __restrict
     r.q = (T)q[n];
     r.lo = (w0 % tl.TS) | ((w1 % tl.TS) << 8) | ((w2 % tl.TS) << 16);
     r.orig = n;
-    rec[tile_start[k] + rank[n]] = r;
+    rec[row_start[k] + rank[n]] = r;
 }
 
 template <typename T>
 void launch_scatter(const double* x, const double* q, int N, const Geom& g, const Tiling& tl, const int* key,
-                    const int* rank, const int* tile_start, Rec<T>* rec, hipStream_t s)
+                    const int* rank, const int* row_start, Rec<T>* rec, hipStream_t s)
 {
     if (N <= 0) return;
     hipLaunchKernelGGL(k_scatter<T>, dim3(cdiv(N, 256)), dim3(256), 0, s, x, q, N, g, tl, key, rank,
-                       tile_start, rec);
+                       row_start, rec);
 }
 
 // ================================================================================
-// Tiled spread.  One work item = up to `chunk` particles of one tile.  The
-// work-group zeroes an S^3 LDS sub-grid, each wave takes one particle at a time:
-// lanes 0..P-1 build the three 1-D Gaussian factors (q*d folded into x), lane l
-// owns (j,k) pairs l, l+64, ... of the P^2 window columns and adds
-// X[i]*Y[j]*Z[k] with ds_add for i = 0..P-1.  The sub-grid is then written to
-// the item's partial slot; the halo merge sums slots in fixed order.
+// Tiled spread (generic P, FP32 or FP64).  One work item = up to `chunk` particles
+// of one tile.  The work-group zeroes an S^3 FP64 LDS sub-grid, each wave takes
+// one particle at a time: lanes 0..P-1 build the three 1-D Gaussian factors (q*d
+// folded into x), lane l owns (j,k) pairs l, l+64, ... of the P^2 window columns
+// and adds X[i]*Y[j]*Z[k] with ds_add_f64 for i = 0..P-1.  (The sub-grid is FP64
+// because gfx950 executes ds_add_f32 about 20x slower than ds_add_f64.)  The
+// sub-grid is then written to the item's partial slot; the halo merge sums slots
+// in fixed order.
 // ================================================================================
 // Lane l of a wave owns the window columns (j,k) = divmod(l + 64u, P), u < U,
 // U = ceil(P^2/64).  PT > 0 fixes P at compile time; PT == 0 takes P from the
@@ -225,10 +249,10 @@ __global__ __launch_bounds__(512) void k_spread_tiles(
     const int item = blockIdx.x;
     if (item >= *n_items) return;   // uniform across the work-group
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* sub = reinterpret_cast<T*>(smem_raw);
+    double* sub = reinterpret_cast<double*>(smem_raw);   // FP64 accumulation (ds_add_f64)
     const int P = PT > 0 ? PT : g.P;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int e = tid; e < tl.sub_elems; e += nthr) sub[e] = T(0);
+    for (int e = tid; e < tl.sub_elems; e += nthr) sub[e] = 0.0;
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
@@ -256,14 +280,14 @@ __global__ __launch_bounds__(512) void k_spread_tiles(
             yz[u] = lc.ok(u) ? v : T(0);
         }
         const int lx = r.lo & 255, ly = (r.lo >> 8) & 255, lz = r.lo >> 16;
-        T* base = sub + (lx * tl.S + ly) * tl.Szp + lz;
+        double* base = sub + (lx * tl.S + ly) * tl.Szp + lz;
 #pragma unroll 4
         for (int i = 0; i < P; ++i) {
             const T xi = bcast(fx, i);
-            T* row = base + i * slab;
+            double* row = base + i * slab;
 #pragma unroll
             for (int u = 0; u < UM; ++u)
-                if (lc.ok(u)) lds_add(row + off[u], xi * yz[u]);
+                if (lc.ok(u)) lds_add(row + off[u], (double)(xi * yz[u]));
         }
     }
     __syncthreads();
@@ -271,7 +295,7 @@ __global__ __launch_bounds__(512) void k_spread_tiles(
     const int S = tl.S;
     for (int e = tid; e < tl.part_elems; e += nthr) {
         int iz = e % S, r2 = e / S, iy = r2 % S, ix = r2 / S;
-        out[e] = sub[(ix * S + iy) * tl.Szp + iz];
+        out[e] = (T)sub[(ix * S + iy) * tl.Szp + iz];
     }
 }
 
@@ -279,7 +303,7 @@ template <typename T>
 void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                          const int* n_items, const Geom& g, const Tiling& tl, T* partials, hipStream_t s)
 {
-    size_t lds = (size_t)tl.sub_elems * sizeof(T);
+    size_t lds = (size_t)tl.sub_elems * sizeof(double);
     dim3 grid(tl.max_items), block(tl.block);
 #define SE_SPREAD(PT, UM)                                                                                   \
     do {                                                                                                    \

@@ -105,9 +105,9 @@ def test_host_parameter_routines():
 def test_plan_rejects_bad_descriptions_without_gpu_compute():
     # argument validation happens before any device work
     lib = se.capi()
-    d = se.se_plan_desc(3, 0, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0)
+    d = se.se_plan_desc(3, 0, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0, 0)
     h = C.c_void_p()
     assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
-    d = se.se_plan_desc(2, 32, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0)
+    d = se.se_plan_desc(2, 32, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0, 0)
     assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
     assert "periodicity" in se.last_error()
