@@ -40,26 +40,86 @@ __device__ __forceinline__ int modp(int v)
     return r < 0 ? r + P : r;
 }
 
-__device__ __forceinline__ float wave_sum_f(float v)
+// exp(-c a^2) as a native v_exp_f32 of kc * a^2, kc = -c log2(e) (about 1 ulp; the
+// argument stays far above the FP32 denormal range for the supports used here)
+__device__ __forceinline__ float gauss(float kc, float a) { return __builtin_amdgcn_exp2f(kc * a * a); }
+
+// Particle records are fetched 64 at a time with one coalesced vector load (lane l holds
+// record b0 + l) and the fields of particle b0 + i are read into SGPRs with v_readlane.
+// Scalar loads would be simpler, but SMEM shares the lgkmcnt counter with LDS and
+// returns out of order, so every wait on an LDS result would also wait for them.
+struct LaneRec {
+    float4 a;   // t[0], t[1], t[2], q
+    int4 b;     // lo, orig, pad, pad
+};
+
+__device__ __forceinline__ LaneRec load_lane_rec(const Rec<float>* rec, int i)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    const float4* p = reinterpret_cast<const float4*>(rec + i);
+    LaneRec r;
+    r.a = p[0];
+    r.b = *reinterpret_cast<const int4*>(p + 1);
+    return r;
+}
+
+__device__ __forceinline__ Rec<float> pick_rec(const LaneRec& s, int l)
+{
+    Rec<float> r;
+    r.t[0] = rlane(s.a.x, l);
+    r.t[1] = rlane(s.a.y, l);
+    r.t[2] = rlane(s.a.z, l);
+    r.q = rlane(s.a.w, l);
+    r.lo = __builtin_amdgcn_readlane(s.b.x, l);
+    r.orig = __builtin_amdgcn_readlane(s.b.y, l);
+    return r;
+}
+
+// Slot-rotated window row: lane s of the ring holds row rho == s (mod P) of the window
+// starting at row y, i.e. window index (s - y) mod P; ymodp = y mod P (wave-uniform).
+template <int P>
+__device__ __forceinline__ int ring_index(int lane, int ymodp)
+{
+    const int j = lane - ymodp;
+    return j < 0 ? j + P : j;
+}
+
+// v += DPP-permuted v (the permuted operand of lanes outside row_mask reads 0)
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_add(float v)
+{
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+// Wave sum with DPP: quad swaps, half-row and row mirrors, then row broadcasts 15 and
+// 31 (GFX9 DPP); lane 63 ends with the total, which is read into an SGPR.
+__device__ __forceinline__ float wave_sum_dpp(float v)
+{
+    v = dpp_add<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+    v = dpp_add<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+    v = dpp_add<0x141, 0xF>(v);   // row_half_mirror
+    v = dpp_add<0x140, 0xF>(v);   // row_mirror
+    v = dpp_add<0x142, 0xA>(v);   // row_bcast:15 into rows 1, 3
+    v = dpp_add<0x143, 0xC>(v);   // row_bcast:31 into rows 2, 3
+    return rlane(v, 63);
 }
 
 // Lane-owned (i, k) pairs of the (P+1)^2 pencil block: offset i*S + k in an LDS row
-// plane, or -1 for padding slots.
+// plane.  Padding slots (pair index >= (P+1)^2, only in the last register) are mapped
+// onto pairs of the first register, so every lane addresses a distinct, in-bounds
+// location without a branch; their weights are zero (ok[r] = false).
 template <int P, int R>
-__device__ __forceinline__ void lane_pairs(int lane, int S, int (&pi)[R], int (&pk)[R], int (&pofs)[R])
+__device__ __forceinline__ void lane_pairs(int lane, int S, int (&pi)[R], int (&pk)[R], int (&pofs)[R],
+                                           bool (&ok)[R])
 {
     constexpr int W = P + 1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int pr = lane + 64 * r;
-        const bool ok = pr < W * W;
-        pi[r] = ok ? pr / W : 0;
-        pk[r] = ok ? pr - (pr / W) * W : 0;
-        pofs[r] = ok ? pi[r] * S + pk[r] : -1;
+        const int pr0 = lane + 64 * r;
+        ok[r] = pr0 < W * W;
+        const int pr = ok[r] ? pr0 : pr0 % (W * W);
+        pi[r] = pr / W;
+        pk[r] = pr - (pr / W) * W;
+        pofs[r] = pi[r] * S + pk[r];
     }
 }
 
@@ -83,9 +143,12 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
     for (int e = tid; e < S3; e += nthr) sub[e] = 0.0;
     __syncthreads();
 
-    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+    // wave index made provably uniform, so particle records are scalar loads and all
+    // ring bookkeeping (pencil, base row, flush tests) lives in SGPRs
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
     int pi[R], pk[R], pofs[R];
-    lane_pairs<P, R>(lane, S, pi, pk, pofs);
+    bool ok[R];
+    lane_pairs<P, R>(lane, S, pi, pk, pofs, ok);
 
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
     const int wb = beg + (int)(((long)n * wave) / nw), we = beg + (int)(((long)n * (wave + 1)) / nw);
@@ -107,50 +170,77 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
                 double* rowp = sub + rho * S2 + pbase;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    if (pofs[r] >= 0)
-                        __hip_atomic_fetch_add(rowp + pofs[r], (double)acc[s][r], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    // padding slots hold 0 and add it to a valid, distinct location
+                    __hip_atomic_fetch_add(rowp + pofs[r], (double)acc[s][r], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                     acc[s][r] = 0.f;
                 }
             }
         }
     };
 
+    // Factors of one particle: x*z products for the lane's pairs (q*d folded in) and the
+    // slot-rotated y factor (lane s < P holds Y[(s - y) mod P]).
+    const float kc = -c * 1.4426950408889634f;
+    const float olane = (float)(g.first + lane) * h;   // grid offset of window index `lane`
+    auto factors = [&](const Rec<float>& rc, float (&xz)[R], float& fy) {
+        const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
+        const int dx = lx & 1, dz = lz & 1;
+        // block column `lane` is window index lane - d (d = the particle's column in the pencil)
+        const float ax = rc.t[0] - olane + (float)dx * h, az = rc.t[2] - olane + (float)dz * h;
+        const bool vx = lane >= dx && lane < P + dx, vz = lane >= dz && lane < P + dz;
+        const float fx = vx ? gauss(kc, ax) * (rc.q * dn) : 0.f;
+        const float fz = vz ? gauss(kc, az) : 0.f;
+        const float ay = rc.t[1] - (float)(g.first + ring_index<P>(lane, ly % P)) * h;
+        fy = gauss(kc, ay);   // lanes >= P are never read
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float v = __shfl(fx, pi[r], 64) * __shfl(fz, pk[r], 64);
+            xz[r] = ok[r] ? v : 0.f;
+        }
+    };
+
+    // Two-stage pipeline: while the FMA block of particle p runs, the record of p+2 is
+    // loading and the factors (exp + cross-lane shuffles) of p+1 are in flight.
+    float xz_n[R], fy_n = 0.f;
+    int b0 = wb;
+    LaneRec cur = load_lane_rec(rec, min(b0 + lane, max(we - 1, beg)));
+    LaneRec nxt = load_lane_rec(rec, min(b0 + 64 + lane, max(we - 1, beg)));
+    Rec<float> rc_n = pick_rec(cur, 0);
+    if (wb < we) factors(rc_n, xz_n, fy_n);
     for (int p = wb; p < we; ++p) {
-        const Rec<float> rc = rec[p];
+        if (p + 1 - b0 == 64) {   // the look-ahead record starts the next batch of 64
+            b0 += 64;
+            cur = nxt;
+            nxt = load_lane_rec(rec, min(b0 + 64 + lane, we - 1));
+        }
+        const Rec<float> rc = rc_n;
+        float xz[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) xz[r] = xz_n[r];
+        const float fy = fy_n;
+        if (p + 1 < we) {
+            rc_n = pick_rec(cur, p + 1 - b0);
+            factors(rc_n, xz_n, fy_n);
+        }
+
         const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
         const int npen = ((lx >> 1) << 12) | (lz >> 1);
+        // one flush site (rows [base, upto) of the old window), so the accumulator ring
+        // is never copied between control-flow paths
+        int upto = base;
         if (npen != pen) {
-            if (pen >= 0) flush(base + P);
+            if (pen >= 0) upto = base + P;
+        } else if (ly > base) {
+            upto = min(ly, base + P);
+        }
+        flush(upto);
+        if (npen != pen) {
             pen = npen;
             base = ly;
             pbase = ((lx >> 1) * 2) * S + (lz >> 1) * 2;
         } else if (ly > base) {
-            flush(min(ly, base + P));
             base = ly;
-        }
-        const int dx = lx & 1, dz = lz & 1;
-        float fx = 0.f, fz = 0.f, fy = 0.f;
-        if (lane < W) {
-            const int ix = lane - dx, iz = lane - dz;
-            if (ix >= 0 && ix < P) {
-                const float a = rc.t[0] - (float)(g.first + ix) * h;
-                fx = expf(-c * a * a) * (rc.q * dn);
-            }
-            if (iz >= 0 && iz < P) {
-                const float a = rc.t[2] - (float)(g.first + iz) * h;
-                fz = expf(-c * a * a);
-            }
-        }
-        if (lane < P) {
-            const float a = rc.t[1] - (float)(g.first + modp<P>(lane - ly)) * h;
-            fy = expf(-c * a * a);
-        }
-        float xz[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float v = __shfl(fx, pi[r], 64) * __shfl(fz, pk[r], 64);
-            xz[r] = pofs[r] >= 0 ? v : 0.f;
         }
 #pragma unroll
         for (int s = 0; s < P; ++s) {
@@ -196,9 +286,10 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
     }
     __syncthreads();
 
-    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
     int pi[R], pk[R], pofs[R];
-    lane_pairs<P, R>(lane, S, pi, pk, pofs);
+    bool ok[R];
+    lane_pairs<P, R>(lane, S, pi, pk, pofs, ok);
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
     const int wb = beg + (int)(((long)n * wave) / nw), we = beg + (int)(((long)n * (wave + 1)) / nw);
     const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
@@ -214,46 +305,99 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
             if (rho >= lo && rho < hi) {
                 const float* rowp = sub + rho * S2 + pbase;
 #pragma unroll
-                for (int r = 0; r < R; ++r) G[s][r] = pofs[r] >= 0 ? rowp[pofs[r]] : 0.f;
+                for (int r = 0; r < R; ++r) G[s][r] = rowp[pofs[r]];   // padding slots: weight 0
             }
         }
     };
 
+    // Factors of one particle: x*z products for the lane's pairs (d folded in), the
+    // slot-rotated y factor and its derivative factor -c (t_y - o h) Y.
+    const float kc = -c * 1.4426950408889634f;
+    const float olane = (float)(g.first + lane) * h;   // grid offset of window index `lane`
+    float oxr[R], ozr[R];                              // grid offsets of the lane's pair columns
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        oxr[r] = (float)(g.first + pi[r]) * h;
+        ozr[r] = (float)(g.first + pk[r]) * h;
+    }
+    auto factors = [&](const Rec<float>& rc, float (&xz)[R], float& fy, float& fyd) {
+        const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
+        const int dx = lx & 1, dz = lz & 1;
+        const float ax = rc.t[0] - olane + (float)dx * h, az = rc.t[2] - olane + (float)dz * h;
+        const bool vx = lane >= dx && lane < P + dx, vz = lane >= dz && lane < P + dz;
+        const float fx = vx ? gauss(kc, ax) * dn : 0.f;
+        const float fz = vz ? gauss(kc, az) : 0.f;
+        const float ay = rc.t[1] - (float)(g.first + ring_index<P>(lane, ly % P)) * h;
+        fy = gauss(kc, ay);   // lanes >= P are never read
+        fyd = fy * (-c * ay);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float v = __shfl(fx, pi[r], 64) * __shfl(fz, pk[r], 64);
+            xz[r] = ok[r] ? v : 0.f;
+        }
+    };
+    // wave sums of the previous particle's lane partials, then its outputs
+    auto finish = [&](float sp, float s0, float s1, float s2, int o, float q) {
+        if (POT) sp = wave_sum_dpp(sp);
+        if (FORCE) {
+            s0 = wave_sum_dpp(s0);
+            s1 = wave_sum_dpp(s1);
+            s2 = wave_sum_dpp(s2);
+        }
+        if (lane == 0) {
+            if (POT) phi[o] = h3 * (double)sp;
+            if (FORCE) {
+                const double qh = h3 * (double)q;
+                force[o] = qh * (double)s0;
+                force[o + (long)N] = qh * (double)s1;
+                force[o + 2l * N] = qh * (double)s2;
+            }
+        }
+    };
+
+    // Pipeline: record of p+2 loading, factors of p+1 in flight and the reduction of
+    // p-1 run alongside the FMA block of particle p.
+    float xz_n[R], fy_n = 0.f, fyd_n = 0.f;
+    int b0 = wb;
+    LaneRec cur = load_lane_rec(rec, min(b0 + lane, max(we - 1, beg)));
+    LaneRec nxt = load_lane_rec(rec, min(b0 + 64 + lane, max(we - 1, beg)));
+    Rec<float> rc_n = pick_rec(cur, 0);
+    if (wb < we) factors(rc_n, xz_n, fy_n, fyd_n);
+    float psp = 0.f, ps0 = 0.f, ps1 = 0.f, ps2 = 0.f, pq = 0.f;
+    int porig = -1;
     for (int p = wb; p < we; ++p) {
-        const Rec<float> rc = rec[p];
+        if (p + 1 - b0 == 64) {   // the look-ahead record starts the next batch of 64
+            b0 += 64;
+            cur = nxt;
+            nxt = load_lane_rec(rec, min(b0 + 64 + lane, we - 1));
+        }
+        const Rec<float> rc = rc_n;
+        float xz[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) xz[r] = xz_n[r];
+        const float fy = fy_n, fyd = fyd_n;
+        if (p + 1 < we) {
+            rc_n = pick_rec(cur, p + 1 - b0);
+            factors(rc_n, xz_n, fy_n, fyd_n);
+        }
+
         const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
         const int npen = ((lx >> 1) << 12) | (lz >> 1);
+        // one load site (rows [lo, hi) of the new window) for the same reason as the
+        // spread's single flush site
+        int lo = 0, hi = 0;
         if (npen != pen) {
             pen = npen;
-            base = ly;
             pbase = ((lx >> 1) * 2) * S + (lz >> 1) * 2;
-            load(base, base + P);
-        } else if (ly > base) {
-            const int old_end = base + P;
+            lo = ly;
+            hi = ly + P;
             base = ly;
-            load(max(old_end, ly), ly + P);
+        } else if (ly > base) {
+            lo = max(base + P, ly);
+            hi = ly + P;
+            base = ly;
         }
-        const int dx = lx & 1, dz = lz & 1;
-        float fx = 0.f, fz = 0.f, fy = 0.f, gx = 0.f, gz = 0.f, gy = 0.f;   // factors and -c*(t - o h)
-        if (lane < W) {
-            const int ix = lane - dx, iz = lane - dz;
-            if (ix >= 0 && ix < P) {
-                const float a = rc.t[0] - (float)(g.first + ix) * h;
-                fx = expf(-c * a * a) * dn;
-                gx = -c * a;
-            }
-            if (iz >= 0 && iz < P) {
-                const float a = rc.t[2] - (float)(g.first + iz) * h;
-                fz = expf(-c * a * a);
-                gz = -c * a;
-            }
-        }
-        if (lane < P) {
-            const float a = rc.t[1] - (float)(g.first + modp<P>(lane - ly)) * h;
-            fy = expf(-c * a * a);
-            gy = -c * a;
-        }
-        const float fyd = fy * gy;
+        load(lo, hi);
         float A[R], B[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) A[r] = B[r] = 0.f;
@@ -267,36 +411,29 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
                 if (FORCE) B[r] = fmaf(yd, G[s][r], B[r]);
             }
         }
+        if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq);
+        // lane partials of particle p; x/z derivative factors -c (t - o h) from the pair index
+        const float tx = rc.t[0] + (float)(lx & 1) * h, tz = rc.t[2] + (float)(lz & 1) * h;
         float sp = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const float xv = __shfl(fx, pi[r], 64), zv = __shfl(fz, pk[r], 64);
-            const float xz = pofs[r] >= 0 ? xv * zv : 0.f;
-            if (POT || FORCE) sp = fmaf(xz, A[r], sp);
+            sp = fmaf(xz[r], A[r], sp);
             if (FORCE) {
-                const float xg = __shfl(gx, pi[r], 64), zg = __shfl(gz, pk[r], 64);
-                s0 = fmaf(xz * xg, A[r], s0);
-                s1 = fmaf(xz, B[r], s1);
-                s2 = fmaf(xz * zg, A[r], s2);
+                const float xg = -c * (tx - oxr[r]);
+                const float zg = -c * (tz - ozr[r]);
+                s0 = fmaf(xz[r] * xg, A[r], s0);
+                s1 = fmaf(xz[r], B[r], s1);
+                s2 = fmaf(xz[r] * zg, A[r], s2);
             }
         }
-        if (POT) sp = wave_sum_f(sp);
-        if (FORCE) {
-            s0 = wave_sum_f(s0);
-            s1 = wave_sum_f(s1);
-            s2 = wave_sum_f(s2);
-        }
-        if (lane == 0) {
-            const int o = rc.orig;
-            if (POT) phi[o] = h3 * (double)sp;
-            if (FORCE) {
-                const double qh = h3 * (double)rc.q;
-                force[o] = qh * (double)s0;
-                force[o + (long)N] = qh * (double)s1;
-                force[o + 2l * N] = qh * (double)s2;
-            }
-        }
+        psp = sp;
+        ps0 = s0;
+        ps1 = s1;
+        ps2 = s2;
+        porig = rc.orig;
+        pq = rc.q;
     }
+    if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq);
 }
 
 bool pencil_supported(int P) { return P == 8 || P == 11 || P == 16; }
