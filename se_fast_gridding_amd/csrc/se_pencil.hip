@@ -40,6 +40,16 @@ __device__ __forceinline__ int modp(int v)
     return r < 0 ? r + P : r;
 }
 
+// Ring slots holding rows [lo, lo + n) (0 <= n <= P) as a bit mask: slot s holds the
+// row == s (mod P), so the mask is n consecutive bits rotated by lo mod P.
+template <int P>
+__device__ __forceinline__ unsigned ring_mask(int lo, int n)
+{
+    const unsigned m = n >= P ? (1u << P) - 1u : (1u << n) - 1u;
+    const int b = lo % P;
+    return ((m << b) | (m >> (P - b))) & ((1u << P) - 1u);
+}
+
 // exp(-c a^2) as a native v_exp_f32 of kc * a^2, kc = -c log2(e) (about 1 ulp; the
 // argument stays far above the FP32 denormal range for the supports used here)
 __device__ __forceinline__ float gauss(float kc, float a) { return __builtin_amdgcn_exp2f(kc * a * a); }
@@ -104,23 +114,33 @@ __device__ __forceinline__ float wave_sum_dpp(float v)
 }
 
 // Lane-owned (i, k) pairs of the (P+1)^2 pencil block: offset i*S + k in an LDS row
-// plane.  Padding slots (pair index >= (P+1)^2, only in the last register) are mapped
-// onto pairs of the first register, so every lane addresses a distinct, in-bounds
-// location without a branch; their weights are zero (ok[r] = false).
+// plane, and the ds_bpermute byte addresses (4*i, 4*k) that fetch the pair's x and z
+// factors from the lanes that computed them.  Padding slots (pair index >= (P+1)^2,
+// only in the last register) are mapped onto pairs of the first register, so every
+// lane addresses a distinct, in-bounds location without a branch, and fetch their x
+// factor from lane 63, whose factor is always 0 (63 >= P + 1).
 template <int P, int R>
 __device__ __forceinline__ void lane_pairs(int lane, int S, int (&pi)[R], int (&pk)[R], int (&pofs)[R],
-                                           bool (&ok)[R])
+                                           int (&bx)[R], int (&bz)[R])
 {
     constexpr int W = P + 1;
+    static_assert(W < 63, "lane 63 must lie outside the window");
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int pr0 = lane + 64 * r;
-        ok[r] = pr0 < W * W;
-        const int pr = ok[r] ? pr0 : pr0 % (W * W);
+        const bool ok = pr0 < W * W;
+        const int pr = ok ? pr0 : pr0 % (W * W);
         pi[r] = pr / W;
         pk[r] = pr - (pr / W) * W;
         pofs[r] = pi[r] * S + pk[r];
+        bx[r] = 4 * (ok ? pi[r] : 63);
+        bz[r] = 4 * pk[r];
     }
+}
+
+__device__ __forceinline__ float bperm(int byte_addr, float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(byte_addr, __float_as_int(v)));
 }
 
 }  // namespace
@@ -143,12 +163,11 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
     for (int e = tid; e < S3; e += nthr) sub[e] = 0.0;
     __syncthreads();
 
-    // wave index made provably uniform, so particle records are scalar loads and all
-    // ring bookkeeping (pencil, base row, flush tests) lives in SGPRs
+    // wave index made provably uniform, so the particle range and all ring bookkeeping
+    // (pencil, base row, flush tests) live in SGPRs
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
-    int pi[R], pk[R], pofs[R];
-    bool ok[R];
-    lane_pairs<P, R>(lane, S, pi, pk, pofs, ok);
+    int pi[R], pk[R], pofs[R], bx[R], bz[R];
+    lane_pairs<P, R>(lane, S, pi, pk, pofs, bx, bz);
 
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
     const int wb = beg + (int)(((long)n * wave) / nw), we = beg + (int)(((long)n * (wave + 1)) / nw);
@@ -161,12 +180,16 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         for (int r = 0; r < R; ++r) acc[s][r] = 0.f;
 
     int pen = -1, base = 0, pbase = 0;
-    // add ring rows [base, upto) (upto <= base + P) into LDS and clear their slots
+    // add ring rows [base, upto) (upto <= base + P) into LDS and clear their slots; the
+    // common case (no row leaves the window) costs one uniform branch
     auto flush = [&](int upto) {
+        if (upto <= base) return;
+        const unsigned mask = ring_mask<P>(base, upto - base);
+        const int b = base % P;
 #pragma unroll
         for (int s = 0; s < P; ++s) {
-            const int rho = base + modp<P>(s - base);
-            if (rho < upto) {
+            if (mask & (1u << s)) {
+                const int rho = base + (s >= b ? s - b : s - b + P);
                 double* rowp = sub + rho * S2 + pbase;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -179,35 +202,40 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         }
     };
 
-    // Factors of one particle: x*z products for the lane's pairs (q*d folded in) and the
-    // slot-rotated y factor (lane s < P holds Y[(s - y) mod P]).
+    // Factors of one particle: the x and z factors of the lane's pairs (q*d folded into
+    // x), fetched with ds_bpermute and multiplied only when the particle is processed,
+    // and the slot-rotated y factor (lane s < P holds Y[(s - y) mod P]).
     const float kc = -c * 1.4426950408889634f;
     const float olane = (float)(g.first + lane) * h;   // grid offset of window index `lane`
-    auto factors = [&](const Rec<float>& rc, float (&xz)[R], float& fy) {
+    auto factors = [&](const Rec<float>& rc, float (&sx)[R], float (&sz)[R], float& fy) {
         const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
         const int dx = lx & 1, dz = lz & 1;
         // block column `lane` is window index lane - d (d = the particle's column in the pencil)
         const float ax = rc.t[0] - olane + (float)dx * h, az = rc.t[2] - olane + (float)dz * h;
         const bool vx = lane >= dx && lane < P + dx, vz = lane >= dz && lane < P + dz;
-        const float fx = vx ? gauss(kc, ax) * (rc.q * dn) : 0.f;
-        const float fz = vz ? gauss(kc, az) : 0.f;
+        // the exponentials run on every lane and the masks select afterwards, so no
+        // exec-mask branch is formed around them
+        const float fx = gauss(kc, ax) * (vx ? rc.q * dn : 0.f);
+        const float fz = gauss(kc, az) * (vz ? 1.f : 0.f);
         const float ay = rc.t[1] - (float)(g.first + ring_index<P>(lane, ly % P)) * h;
         fy = gauss(kc, ay);   // lanes >= P are never read
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const float v = __shfl(fx, pi[r], 64) * __shfl(fz, pk[r], 64);
-            xz[r] = ok[r] ? v : 0.f;
+            sx[r] = bperm(bx[r], fx);
+            sz[r] = bperm(bz[r], fz);
         }
     };
 
-    // Two-stage pipeline: while the FMA block of particle p runs, the record of p+2 is
-    // loading and the factors (exp + cross-lane shuffles) of p+1 are in flight.
-    float xz_n[R], fy_n = 0.f;
+    // Pipeline: the record of particle p+1 is picked and its factors (exponentials and
+    // cross-lane fetches) are issued before the FMA block of particle p, and their LDS
+    // results are first used one iteration later, so the fetch latency overlaps the FMAs.
+    // The look-ahead at the last particle reads a clamped (valid) record and is unused.
+    float sx_n[R], sz_n[R], fy_n = 0.f;
     int b0 = wb;
     LaneRec cur = load_lane_rec(rec, min(b0 + lane, max(we - 1, beg)));
     LaneRec nxt = load_lane_rec(rec, min(b0 + 64 + lane, max(we - 1, beg)));
     Rec<float> rc_n = pick_rec(cur, 0);
-    if (wb < we) factors(rc_n, xz_n, fy_n);
+    factors(rc_n, sx_n, sz_n, fy_n);
     for (int p = wb; p < we; ++p) {
         if (p + 1 - b0 == 64) {   // the look-ahead record starts the next batch of 64
             b0 += 64;
@@ -217,12 +245,13 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         const Rec<float> rc = rc_n;
         float xz[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) xz[r] = xz_n[r];
-        const float fy = fy_n;
-        if (p + 1 < we) {
-            rc_n = pick_rec(cur, p + 1 - b0);
-            factors(rc_n, xz_n, fy_n);
+        for (int r = 0; r < R; ++r) {
+            xz[r] = sx_n[r] * sz_n[r];
+            // form the products here, before the flush's LDS atomics are issued: the wait
+            // for the fetches then does not also wait for those atomics
+            asm volatile("" : "+v"(xz[r]));
         }
+        const float fy = fy_n;
 
         const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
         const int npen = ((lx >> 1) << 12) | (lz >> 1);
@@ -234,6 +263,11 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         } else if (ly > base) {
             upto = min(ly, base + P);
         }
+        // y factors into SGPRs ahead of the flush, so the readlane->FMA wait states are
+        // covered by other work
+        float ys[P];
+#pragma unroll
+        for (int s = 0; s < P; ++s) ys[s] = rlane(fy, s);
         flush(upto);
         if (npen != pen) {
             pen = npen;
@@ -242,12 +276,12 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         } else if (ly > base) {
             base = ly;
         }
+        rc_n = pick_rec(cur, p + 1 - b0);
+        factors(rc_n, sx_n, sz_n, fy_n);
 #pragma unroll
-        for (int s = 0; s < P; ++s) {
-            const float ys = rlane(fy, s);
+        for (int s = 0; s < P; ++s)
 #pragma unroll
-            for (int r = 0; r < R; ++r) acc[s][r] = fmaf(ys, xz[r], acc[s][r]);
-        }
+            for (int r = 0; r < R; ++r) acc[s][r] = fmaf(ys[s], xz[r], acc[s][r]);
     }
     if (pen >= 0) flush(base + P);
     __syncthreads();
@@ -287,9 +321,8 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
     __syncthreads();
 
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
-    int pi[R], pk[R], pofs[R];
-    bool ok[R];
-    lane_pairs<P, R>(lane, S, pi, pk, pofs, ok);
+    int pi[R], pk[R], pofs[R], bx[R], bz[R];
+    lane_pairs<P, R>(lane, S, pi, pk, pofs, bx, bz);
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
     const int wb = beg + (int)(((long)n * wave) / nw), we = beg + (int)(((long)n * (wave + 1)) / nw);
     const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
@@ -297,12 +330,16 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
 
     float G[P][R];
     int pen = -1, base = 0, pbase = 0;
-    // read ring rows in [lo, hi) (all within [base, base + P)) from LDS
+    // read ring rows in [lo, hi) (all within [base, base + P)) from LDS; one uniform
+    // branch when no row enters the window
     auto load = [&](int lo, int hi) {
+        if (hi <= lo) return;
+        const unsigned mask = ring_mask<P>(lo, hi - lo);
+        const int b = base % P;
 #pragma unroll
         for (int s = 0; s < P; ++s) {
-            const int rho = base + modp<P>(s - base);
-            if (rho >= lo && rho < hi) {
+            if (mask & (1u << s)) {
+                const int rho = base + (s >= b ? s - b : s - b + P);
                 const float* rowp = sub + rho * S2 + pbase;
 #pragma unroll
                 for (int r = 0; r < R; ++r) G[s][r] = rowp[pofs[r]];   // padding slots: weight 0
@@ -310,35 +347,39 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
         }
     };
 
-    // Factors of one particle: x*z products for the lane's pairs (d folded in), the
+    // Factors of one particle: the x and z factors of the lane's pairs (d folded into x),
+    // fetched with ds_bpermute and multiplied only when the particle is processed, the
     // slot-rotated y factor and its derivative factor -c (t_y - o h) Y.
     const float kc = -c * 1.4426950408889634f;
     const float olane = (float)(g.first + lane) * h;   // grid offset of window index `lane`
-    float oxr[R], ozr[R];                              // grid offsets of the lane's pair columns
+    // c * (grid offset) of the lane's pair columns: the x/z derivative factor of a pair is
+    // -c (t - o h) = c o h - c t, and the -c t part is applied once per particle
+    float coxr[R], cozr[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        oxr[r] = (float)(g.first + pi[r]) * h;
-        ozr[r] = (float)(g.first + pk[r]) * h;
+        coxr[r] = c * ((float)(g.first + pi[r]) * h);
+        cozr[r] = c * ((float)(g.first + pk[r]) * h);
     }
-    auto factors = [&](const Rec<float>& rc, float (&xz)[R], float& fy, float& fyd) {
+    auto factors = [&](const Rec<float>& rc, float (&sx)[R], float (&sz)[R], float& fy, float& fyd) {
         const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
         const int dx = lx & 1, dz = lz & 1;
         const float ax = rc.t[0] - olane + (float)dx * h, az = rc.t[2] - olane + (float)dz * h;
         const bool vx = lane >= dx && lane < P + dx, vz = lane >= dz && lane < P + dz;
-        const float fx = vx ? gauss(kc, ax) * dn : 0.f;
-        const float fz = vz ? gauss(kc, az) : 0.f;
+        const float fx = gauss(kc, ax) * (vx ? dn : 0.f);
+        const float fz = gauss(kc, az) * (vz ? 1.f : 0.f);
         const float ay = rc.t[1] - (float)(g.first + ring_index<P>(lane, ly % P)) * h;
         fy = gauss(kc, ay);   // lanes >= P are never read
         fyd = fy * (-c * ay);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const float v = __shfl(fx, pi[r], 64) * __shfl(fz, pk[r], 64);
-            xz[r] = ok[r] ? v : 0.f;
+            sx[r] = bperm(bx[r], fx);
+            sz[r] = bperm(bz[r], fz);
         }
     };
-    // wave sums of the previous particle's lane partials, then its outputs
-    auto finish = [&](float sp, float s0, float s1, float s2, int o, float q) {
-        if (POT) sp = wave_sum_dpp(sp);
+    // wave sums of the previous particle's lane partials, then its outputs; ctx, ctz are
+    // c * (the particle's t_x, t_z in block-column coordinates)
+    auto finish = [&](float sp, float s0, float s1, float s2, int o, float q, float ctx, float ctz) {
+        sp = wave_sum_dpp(sp);
         if (FORCE) {
             s0 = wave_sum_dpp(s0);
             s1 = wave_sum_dpp(s1);
@@ -348,22 +389,24 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
             if (POT) phi[o] = h3 * (double)sp;
             if (FORCE) {
                 const double qh = h3 * (double)q;
-                force[o] = qh * (double)s0;
+                force[o] = qh * ((double)s0 - (double)ctx * (double)sp);
                 force[o + (long)N] = qh * (double)s1;
-                force[o + 2l * N] = qh * (double)s2;
+                force[o + 2l * N] = qh * ((double)s2 - (double)ctz * (double)sp);
             }
         }
     };
 
-    // Pipeline: record of p+2 loading, factors of p+1 in flight and the reduction of
-    // p-1 run alongside the FMA block of particle p.
-    float xz_n[R], fy_n = 0.f, fyd_n = 0.f;
+    // Pipeline: the record of particle p+1 is picked and its factors are issued before the
+    // FMA block of particle p (their LDS results are first used one iteration later), and
+    // the wave reductions of particle p-1 run after it.  The look-ahead at the last
+    // particle reads a clamped (valid) record and is unused.
+    float sx_n[R], sz_n[R], fy_n = 0.f, fyd_n = 0.f;
     int b0 = wb;
     LaneRec cur = load_lane_rec(rec, min(b0 + lane, max(we - 1, beg)));
     LaneRec nxt = load_lane_rec(rec, min(b0 + 64 + lane, max(we - 1, beg)));
     Rec<float> rc_n = pick_rec(cur, 0);
-    if (wb < we) factors(rc_n, xz_n, fy_n, fyd_n);
-    float psp = 0.f, ps0 = 0.f, ps1 = 0.f, ps2 = 0.f, pq = 0.f;
+    factors(rc_n, sx_n, sz_n, fy_n, fyd_n);
+    float psp = 0.f, ps0 = 0.f, ps1 = 0.f, ps2 = 0.f, pq = 0.f, pctx = 0.f, pctz = 0.f;
     int porig = -1;
     for (int p = wb; p < we; ++p) {
         if (p + 1 - b0 == 64) {   // the look-ahead record starts the next batch of 64
@@ -374,12 +417,8 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
         const Rec<float> rc = rc_n;
         float xz[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) xz[r] = xz_n[r];
+        for (int r = 0; r < R; ++r) xz[r] = sx_n[r] * sz_n[r];
         const float fy = fy_n, fyd = fyd_n;
-        if (p + 1 < we) {
-            rc_n = pick_rec(cur, p + 1 - b0);
-            factors(rc_n, xz_n, fy_n, fyd_n);
-        }
 
         const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
         const int npen = ((lx >> 1) << 12) | (lz >> 1);
@@ -397,33 +436,38 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
             hi = ly + P;
             base = ly;
         }
+        float ys[P], yd[P];
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+            ys[s] = rlane(fy, s);
+            yd[s] = FORCE ? rlane(fyd, s) : 0.f;
+        }
         load(lo, hi);
+        rc_n = pick_rec(cur, p + 1 - b0);
+        factors(rc_n, sx_n, sz_n, fy_n, fyd_n);
         float A[R], B[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) A[r] = B[r] = 0.f;
 #pragma unroll
         for (int s = 0; s < P; ++s) {
-            const float ys = rlane(fy, s);
-            const float yd = FORCE ? rlane(fyd, s) : 0.f;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                A[r] = fmaf(ys, G[s][r], A[r]);
-                if (FORCE) B[r] = fmaf(yd, G[s][r], B[r]);
+                A[r] = fmaf(ys[s], G[s][r], A[r]);
+                if (FORCE) B[r] = fmaf(yd[s], G[s][r], B[r]);
             }
         }
-        if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq);
-        // lane partials of particle p; x/z derivative factors -c (t - o h) from the pair index
-        const float tx = rc.t[0] + (float)(lx & 1) * h, tz = rc.t[2] + (float)(lz & 1) * h;
+        if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq, pctx, pctz);
+        // lane partials of particle p: sum xz A, and for the x/z force the pair-dependent
+        // part sum xz A c o h of the derivative factor (finish subtracts c t sum xz A)
         float sp = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            sp = fmaf(xz[r], A[r], sp);
+            const float w = xz[r] * A[r];
+            sp += w;
             if (FORCE) {
-                const float xg = -c * (tx - oxr[r]);
-                const float zg = -c * (tz - ozr[r]);
-                s0 = fmaf(xz[r] * xg, A[r], s0);
+                s0 = fmaf(w, coxr[r], s0);
                 s1 = fmaf(xz[r], B[r], s1);
-                s2 = fmaf(xz[r] * zg, A[r], s2);
+                s2 = fmaf(w, cozr[r], s2);
             }
         }
         psp = sp;
@@ -432,8 +476,10 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
         ps2 = s2;
         porig = rc.orig;
         pq = rc.q;
+        pctx = c * (rc.t[0] + (float)(lx & 1) * h);
+        pctz = c * (rc.t[2] + (float)(lz & 1) * h);
     }
-    if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq);
+    if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq, pctx, pctz);
 }
 
 bool pencil_supported(int P) { return P == 8 || P == 11 || P == 16; }
