@@ -102,6 +102,14 @@ def run(args):
 
     dev = torch.device("cuda:0")
     N, M, P, xi = args.n, args.m, args.p, args.xi
+    if args.calib_copy_mib > 0:
+        # one elementwise FP32 copy of known size (reads and writes calib_copy_mib MiB),
+        # launched first, so a counter run sees a reference kernel beside the solve
+        src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     x, q = make_system(N)
     xt = torch.tensor(x, device=dev)
     qt = torch.tensor(q, device=dev)
@@ -166,8 +174,9 @@ def run(args):
     plan.close()
 
     cpu = None
-    if args.cpu_sample > 0:
-        cpu = cpu_baseline(M, P, xi, args.cpu_sample)
+    n_cpu = N if args.cpu_sample < 0 else args.cpu_sample
+    if n_cpu > 0:
+        cpu = cpu_baseline(M, P, xi, n_cpu)
 
     out = {
         "metric": "spread+interp particle*P^3 updates/s",
@@ -207,7 +216,10 @@ def main():
     ap.add_argument("--p", type=int, default=16)
     ap.add_argument("--xi", type=float, default=40.0)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-sample", type=int, default=-1,
+                    help="particles in the CPU reference run (-1 = the whole workload, 0 = skip)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="launch one FP32 copy of this many MiB first (HBM counter calibration)")
     ap.add_argument("--no-check", dest="check", action="store_false")
     ap.add_argument("--tile", type=int, default=0, help="tile side (0 = plan default)")
     ap.add_argument("--chunk", type=int, default=0, help="max particles per work item (0 = default)")

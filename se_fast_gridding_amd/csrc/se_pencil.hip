@@ -143,6 +143,60 @@ __device__ __forceinline__ float bperm(int byte_addr, float v)
     return __int_as_float(__builtin_amdgcn_ds_bpermute(byte_addr, __float_as_int(v)));
 }
 
+// acc += y[lane S of this 16-lane row] * x, as one v_fmac_f32_dpp with row_newbcast:S
+// (GFX9.4 DPP: the broadcast source operand is read through the DPP crossbar, so the
+// y-factor of ring slot S needs no v_readlane).  The y operand is read through DPP,
+// which needs two wait states after the VALU write of that VGPR; FIRST places them in
+// front of the first FMA of a block, and the blocks' asm is volatile so that no VALU
+// write of y is scheduled between them.
+template <int S, bool FIRST>
+__device__ __forceinline__ void fmac_row_bcast(float& acc, float y, float x)
+{
+    if (FIRST)
+        asm volatile("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                     : "+v"(acc) : "v"(y), "v"(x), "n"(S));
+    else
+        asm volatile("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                     : "+v"(acc) : "v"(y), "v"(x), "n"(S));
+}
+
+// Spread FMA block: acc[s][r] += Y_s * xz[r] for all ring slots s, Y_s = lane s of fy's row.
+template <int S, int P, int R>
+struct SpreadRing {
+    static __device__ __forceinline__ void run(float (&acc)[P][R], float fy, const float (&xz)[R])
+    {
+        fmac_row_bcast<S, S == 0>(acc[S][0], fy, xz[0]);
+#pragma unroll
+        for (int r = 1; r < R; ++r) fmac_row_bcast<S, false>(acc[S][r], fy, xz[r]);
+        SpreadRing<S + 1, P, R>::run(acc, fy, xz);
+    }
+};
+template <int P, int R>
+struct SpreadRing<P, P, R> {
+    static __device__ __forceinline__ void run(float (&)[P][R], float, const float (&)[R]) {}
+};
+
+// Gather FMA block: A[r] += Y_s G[s][r] and (FORCE) B[r] += Y'_s G[s][r] over the slots.
+template <int S, int P, int R, bool FORCE>
+struct GatherRing {
+    static __device__ __forceinline__ void run(float (&A)[R], float (&B)[R], float fy, float fyd,
+                                               const float (&G)[P][R])
+    {
+        fmac_row_bcast<S, S == 0>(A[0], fy, G[S][0]);
+#pragma unroll
+        for (int r = 1; r < R; ++r) fmac_row_bcast<S, false>(A[r], fy, G[S][r]);
+        if (FORCE) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) fmac_row_bcast<S, false>(B[r], fyd, G[S][r]);
+        }
+        GatherRing<S + 1, P, R, FORCE>::run(A, B, fy, fyd, G);
+    }
+};
+template <int P, int R, bool FORCE>
+struct GatherRing<P, P, R, FORCE> {
+    static __device__ __forceinline__ void run(float (&)[R], float (&)[R], float, float, const float (&)[P][R]) {}
+};
+
 }  // namespace
 
 template <int P>
@@ -217,8 +271,9 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         // exec-mask branch is formed around them
         const float fx = gauss(kc, ax) * (vx ? rc.q * dn : 0.f);
         const float fz = gauss(kc, az) * (vz ? 1.f : 0.f);
-        const float ay = rc.t[1] - (float)(g.first + ring_index<P>(lane, ly % P)) * h;
-        fy = gauss(kc, ay);   // lanes >= P are never read
+        // every 16-lane row holds the slot-rotated y factors (lanes s < P of the row)
+        const float ay = rc.t[1] - (float)(g.first + ring_index<P>(lane & 15, ly % P)) * h;
+        fy = gauss(kc, ay);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             sx[r] = bperm(bx[r], fx);
@@ -263,11 +318,6 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         } else if (ly > base) {
             upto = min(ly, base + P);
         }
-        // y factors into SGPRs ahead of the flush, so the readlane->FMA wait states are
-        // covered by other work
-        float ys[P];
-#pragma unroll
-        for (int s = 0; s < P; ++s) ys[s] = rlane(fy, s);
         flush(upto);
         if (npen != pen) {
             pen = npen;
@@ -278,10 +328,9 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         }
         rc_n = pick_rec(cur, p + 1 - b0);
         factors(rc_n, sx_n, sz_n, fy_n);
-#pragma unroll
-        for (int s = 0; s < P; ++s)
-#pragma unroll
-            for (int r = 0; r < R; ++r) acc[s][r] = fmaf(ys[s], xz[r], acc[s][r]);
+        float fyb = fy;
+        asm volatile("" : "+v"(fyb));   // last definition of the DPP source before the block
+        SpreadRing<0, P, R>::run(acc, fyb, xz);
     }
     if (pen >= 0) flush(base + P);
     __syncthreads();
@@ -353,7 +402,7 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
     const float kc = -c * 1.4426950408889634f;
     const float olane = (float)(g.first + lane) * h;   // grid offset of window index `lane`
     // c * (grid offset) of the lane's pair columns: the x/z derivative factor of a pair is
-    // -c (t - o h) = c o h - c t, and the -c t part is applied once per particle
+    // -c (t - o h) = c o h - c t
     float coxr[R], cozr[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -367,8 +416,9 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
         const bool vx = lane >= dx && lane < P + dx, vz = lane >= dz && lane < P + dz;
         const float fx = gauss(kc, ax) * (vx ? dn : 0.f);
         const float fz = gauss(kc, az) * (vz ? 1.f : 0.f);
-        const float ay = rc.t[1] - (float)(g.first + ring_index<P>(lane, ly % P)) * h;
-        fy = gauss(kc, ay);   // lanes >= P are never read
+        // every 16-lane row holds the slot-rotated y factors (lanes s < P of the row)
+        const float ay = rc.t[1] - (float)(g.first + ring_index<P>(lane & 15, ly % P)) * h;
+        fy = gauss(kc, ay);
         fyd = fy * (-c * ay);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -376,10 +426,9 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
             sz[r] = bperm(bz[r], fz);
         }
     };
-    // wave sums of the previous particle's lane partials, then its outputs; ctx, ctz are
-    // c * (the particle's t_x, t_z in block-column coordinates)
-    auto finish = [&](float sp, float s0, float s1, float s2, int o, float q, float ctx, float ctz) {
-        sp = wave_sum_dpp(sp);
+    // wave sums of the previous particle's lane partials, then its outputs
+    auto finish = [&](float sp, float s0, float s1, float s2, int o, float q) {
+        if (POT) sp = wave_sum_dpp(sp);
         if (FORCE) {
             s0 = wave_sum_dpp(s0);
             s1 = wave_sum_dpp(s1);
@@ -389,9 +438,9 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
             if (POT) phi[o] = h3 * (double)sp;
             if (FORCE) {
                 const double qh = h3 * (double)q;
-                force[o] = qh * ((double)s0 - (double)ctx * (double)sp);
+                force[o] = qh * (double)s0;
                 force[o + (long)N] = qh * (double)s1;
-                force[o + 2l * N] = qh * ((double)s2 - (double)ctz * (double)sp);
+                force[o + 2l * N] = qh * (double)s2;
             }
         }
     };
@@ -406,7 +455,7 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
     LaneRec nxt = load_lane_rec(rec, min(b0 + 64 + lane, max(we - 1, beg)));
     Rec<float> rc_n = pick_rec(cur, 0);
     factors(rc_n, sx_n, sz_n, fy_n, fyd_n);
-    float psp = 0.f, ps0 = 0.f, ps1 = 0.f, ps2 = 0.f, pq = 0.f, pctx = 0.f, pctz = 0.f;
+    float psp = 0.f, ps0 = 0.f, ps1 = 0.f, ps2 = 0.f, pq = 0.f;
     int porig = -1;
     for (int p = wb; p < we; ++p) {
         if (p + 1 - b0 == 64) {   // the look-ahead record starts the next batch of 64
@@ -436,38 +485,29 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
             hi = ly + P;
             base = ly;
         }
-        float ys[P], yd[P];
-#pragma unroll
-        for (int s = 0; s < P; ++s) {
-            ys[s] = rlane(fy, s);
-            yd[s] = FORCE ? rlane(fyd, s) : 0.f;
-        }
         load(lo, hi);
         rc_n = pick_rec(cur, p + 1 - b0);
         factors(rc_n, sx_n, sz_n, fy_n, fyd_n);
         float A[R], B[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) A[r] = B[r] = 0.f;
-#pragma unroll
-        for (int s = 0; s < P; ++s) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                A[r] = fmaf(ys[s], G[s][r], A[r]);
-                if (FORCE) B[r] = fmaf(yd[s], G[s][r], B[r]);
-            }
-        }
-        if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq, pctx, pctz);
-        // lane partials of particle p: sum xz A, and for the x/z force the pair-dependent
-        // part sum xz A c o h of the derivative factor (finish subtracts c t sum xz A)
+        float fyb = fy, fydb = fyd;
+        asm volatile("" : "+v"(fyb), "+v"(fydb));   // last definitions of the DPP sources
+        GatherRing<0, P, R, FORCE>::run(A, B, fyb, fydb, G);
+        if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq);
+        // lane partials of particle p: sum xz A, and for the x/z force sum xz A (c o h - c t)
+        // (the difference is formed per pair: subtracting c t sum xz A once per particle
+        // instead cancels about one more bit)
+        const float ctx = c * (rc.t[0] + (float)(lx & 1) * h), ctz = c * (rc.t[2] + (float)(lz & 1) * h);
         float sp = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const float w = xz[r] * A[r];
             sp += w;
             if (FORCE) {
-                s0 = fmaf(w, coxr[r], s0);
+                s0 = fmaf(w, coxr[r] - ctx, s0);
                 s1 = fmaf(xz[r], B[r], s1);
-                s2 = fmaf(w, cozr[r], s2);
+                s2 = fmaf(w, cozr[r] - ctz, s2);
             }
         }
         psp = sp;
@@ -476,10 +516,8 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
         ps2 = s2;
         porig = rc.orig;
         pq = rc.q;
-        pctx = c * (rc.t[0] + (float)(lx & 1) * h);
-        pctz = c * (rc.t[2] + (float)(lz & 1) * h);
     }
-    if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq, pctx, pctz);
+    if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq);
 }
 
 bool pencil_supported(int P) { return P == 8 || P == 11 || P == 16; }
