@@ -23,7 +23,8 @@
  *   or_se3p: direct k-space sums (two independent forms) to the FGG error floor.
  *   or_direct3p_*: pairwise form (reference's hs-weighted cosine sum) vs
  *     structure-factor form; force vs finite differences of the potential.
- *   or_direct2p_*: pairwise restatement; SE2P k-space solver agreement.
+ *   or_direct2p_fd + or_direct2p_k0: the slab limit of or_direct3p_sf in an
+ *     L x L x Lz box (minus the k_z = 0 quadratic term), to 2e-15 at Lz = 8 L.
  */
 #include <complex.h>
 #include <math.h>

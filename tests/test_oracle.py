@@ -185,6 +185,29 @@ def test_direct3p_force_is_half_q_grad_phi(oracle):
             assert abs(F[d, m] - 0.5 * q[m] * grad) < 1e-6 * max(1.0, abs(F[d, m]))
 
 
+def test_direct2p_is_the_slab_limit_of_direct3p(oracle):
+    """SE2P k-space sum (k != 0 part + k = 0 part, SE_direct.c:256-338) against the
+    3-periodic sum in an L x L x Lz box.  For a neutral system with z in [0, L),
+    phi_2P(x_m) = phi_3P(x_m; Lz) - (2 pi / V) sum_n q_n (z_m - z_n)^2, up to
+    exp(-xi^2 (Lz - L)^2) and exp(-2 pi (Lz - L) / L): the sum over k_z becomes the
+    integral that defines the 2P kernels, and the missing k_z = 0 point of the 3P sum is
+    the quadratic term.  A dropped term, a wrong sign or a wrong 1/|k| breaks it."""
+    O = oracle
+    rng = np.random.default_rng(21)
+    N, xi, Lz = 16, 3.0, 8.0
+    x = rng.random((3, N))
+    q = rng.uniform(-1, 1, N)
+    q -= q.mean()
+    phi2 = O.direct2p(x.reshape(-1), q, (1.0, 1.0, 1.0), xi, 8)
+    phi3 = O.direct3p(x.reshape(-1), q, (1.0, 1.0, Lz), xi, 50)
+    z = x[2]
+    quad = 2 * np.pi / Lz * np.array([np.sum(q * (z[m] - z) ** 2) for m in range(N)])
+    assert rms_rel(phi2, phi3 - quad) < 1e-13
+    # both parts matter at this size: dropping the k = 0 part fails by orders of magnitude
+    fd_only = O.direct2p(x.reshape(-1), q, (1.0, 1.0, 1.0), xi, 8, with_k0=False)
+    assert rms_rel(fd_only, phi3 - quad) > 1e-3
+
+
 def test_se3p_oracle_vs_direct_config1(oracle):
     """N=100 unit cube, q=+-1, P=21, M=32, xi=6: SE vs direct k-space sum (|j| <= 15)."""
     O = oracle
