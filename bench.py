@@ -24,6 +24,15 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector, packed; equal to its FP32 matr
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per dispatch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, factors from a 512 MiB copy kernel);
+# profiles/se3p_solve_n1M_p16_m128_fp32.md holds the counts and the commands.
+MEASURED_TRAFFIC = {
+    # (kernel, N, P, M, precision, tile): bytes per dispatch
+    ("k_interp_pencil", 1_000_000, 16, 128, "fp32", 12): 287.2e6,
+    ("k_spread_pencil", 1_000_000, 16, 128, "fp32", 12): 137.3e6,
+}
+
 
 def make_system(N, seed=2):
     import numpy as np
@@ -155,6 +164,7 @@ def run(args):
     else:
         kname, kflops, kms = "k_interp_" + kind, gather_flops, ms["gather_kernel"]
     achieved = kflops / (kms * 1e-3) / 1e12
+    traffic = MEASURED_TRAFFIC.get((kname, N, P, M, args.precision, info["tile"]))
 
     # accuracy: FP32 P=16 vs FP64 P=21 on the same system (FGG floor ~1e-13 at P=21)
     err = {}
@@ -199,7 +209,8 @@ def run(args):
         "stage_ms": {k: round(v, 4) for k, v in ms.items()},
         "roofline": {"bound": "mfma", "compute": "VALU FP32 FMA (MFMA not used); peak is MI355X FP32",
                      "kernel": kname, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "flops_per_launch": kflops, "kernel_ms": kms, "traffic": None},
+                     "frac": achieved / peak, "flops_per_launch": kflops, "kernel_ms": kms, "traffic": traffic,
+                     "traffic_gbs": (traffic / (kms * 1e-3) / 1e9) if traffic else None},
         "accuracy": err,
         "cpu_baseline": cpu,
     }
