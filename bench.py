@@ -43,7 +43,36 @@ def make_system(N, seed=2):
     return x, q
 
 
-def cpu_baseline(M, P, xi, n_sample, seed=2):
+def make_clustered(N, seed=3, n_clusters=64, frac=0.8, sigma=0.01):
+    """80 % of the charges in 64 isotropic Gaussian clusters (sigma = 0.01 L, centres
+    uniform, wrapped into the box), 20 % uniform; q = +-1 in exact halves, shuffled."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    nc = int(frac * N)
+    centres = rng.random((3, n_clusters))
+    lab = rng.integers(0, n_clusters, nc)
+    xc = (centres[:, lab] + sigma * rng.standard_normal((3, nc))) % 1.0
+    xu = rng.random((3, N - nc))
+    x = np.concatenate([xc, xu], axis=1)
+    x = np.where(x >= 1.0, 0.0, x)   # (-tiny) % 1.0 can round to 1.0
+    q = np.ones(N)
+    q[N // 2:] = -1.0
+    if N % 2:
+        q[-1] = 0.0
+    rng.shuffle(q)
+    return np.ascontiguousarray(x.reshape(-1)), q
+
+
+WORKLOADS = {
+    # name: (N, M, P, xi, generator, description)
+    "uniform": (1_000_000, 128, 16, 40.0, make_system,
+                "N={N} uniform charges in L=1, charges uniform in [-1,1] shifted to zero sum"),
+    "clustered": (4_000_000, 256, 16, 80.0, make_clustered,
+                  "N={N}: 80% in 64 Gaussian clusters (sigma=0.01 L), 20% uniform, q=+-1 net zero"),
+}
+
+
+def cpu_baseline(M, P, xi, n_sample, gen=make_system, what="the same uniform system"):
     """The reference's own gridding core (oracle/_ref, SE_fgg.c built with its -mavx
     flags) on one host core: expand (force factors) + AVX spread + wrap + extend + AVX
     force/potential interpolation on a bounded sample of the same workload."""
@@ -55,7 +84,7 @@ def cpu_baseline(M, P, xi, n_sample, seed=2):
     if not O.ref_available(O.P3):
         return None
     L = O.ref_lib(O.P3)
-    x, q = make_system(n_sample, seed)
+    x, q = gen(n_sample)
     m = 0.95 * np.sqrt(np.pi * P)
     h, w, eta, c = O.parse_params(1.0, M, P, m, xi)
     p = O.SE_FGG_params()
@@ -84,9 +113,9 @@ def cpu_baseline(M, P, xi, n_sample, seed=2):
         "unit": "particle*P^3 updates/s",
         "cores": 1,
         "kind": "reference",
-        "sample": "N=%d of the same uniform system, P=%d, M=%d^3, FP64 force path: expand_all_SSE_force + "
+        "sample": "N=%d of %s, P=%d, M=%d^3, FP64 force path: expand_all_SSE_force + "
                   "grid_split_AVX_dispatch_force + wrap + extend + int_split_AVX_dispatch_force (%.1f s)"
-                  % (n_sample, P, M, total),
+                  % (n_sample, what, P, M, total),
         "stage_s": {k: round(v, 4) for k, v in t.items()},
         "cpu_model": _cpu_model(),
         "nproc": os.cpu_count(),
@@ -110,7 +139,11 @@ def run(args):
     import se_fast_gridding_amd as se
 
     dev = torch.device("cuda:0")
-    N, M, P, xi = args.n, args.m, args.p, args.xi
+    wN, wM, wP, wxi, gen, wdesc = WORKLOADS[args.workload]
+    N = args.n or wN
+    M = args.m or wM
+    P = args.p or wP
+    xi = args.xi or wxi
     if args.calib_copy_mib > 0:
         # one elementwise FP32 copy of known size (reads and writes calib_copy_mib MiB),
         # launched first, so a counter run sees a reference kernel beside the solve
@@ -119,7 +152,7 @@ def run(args):
         torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
-    x, q = make_system(N)
+    x, q = gen(N)
     xt = torch.tensor(x, device=dev)
     qt = torch.tensor(q, device=dev)
     phi = torch.empty(N, dtype=torch.float64, device=dev)
@@ -186,7 +219,7 @@ def run(args):
     cpu = None
     n_cpu = N if args.cpu_sample < 0 else args.cpu_sample
     if n_cpu > 0:
-        cpu = cpu_baseline(M, P, xi, n_cpu)
+        cpu = cpu_baseline(M, P, xi, n_cpu, gen, "the same %s system" % args.workload)
 
     out = {
         "metric": "spread+interp particle*P^3 updates/s",
@@ -201,8 +234,8 @@ def run(args):
         "vs_baseline": None,
         "dtype": "f32" if prec == se.PREC_FP32 else "f64",
         "data": "synthetic",
-        "config": {"workload": "SE3P potential+force, single system: N=%d uniform charges in L=1, zero net "
-                               "charge, P=%d, M=%d^3, xi=%g" % (N, P, M, xi),
+        "config": {"workload": "SE3P potential+force, single system: %s, P=%d, M=%d^3, xi=%g"
+                               % (wdesc.format(N=N), P, M, xi),
                    "N": N, "P": P, "M": M, "xi": xi, "precision": args.precision, "quantities": "phi+force",
                    "tile": info["tile"], "chunk": info["chunk"], "block": info["block"], "work_items": info["items"]},
         "solve_ms": wall * 1e3,
@@ -222,10 +255,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--m", type=int, default=128)
-    ap.add_argument("--p", type=int, default=16)
-    ap.add_argument("--xi", type=float, default=40.0)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="uniform",
+                    help="uniform: N=1M, M=128, xi=40 (the headline line); clustered: N=4M, M=256, xi=80")
+    ap.add_argument("--n", type=int, default=0, help="particles (0 = the workload's)")
+    ap.add_argument("--m", type=int, default=0, help="grid side (0 = the workload's)")
+    ap.add_argument("--p", type=int, default=0, help="Gaussian support (0 = the workload's)")
+    ap.add_argument("--xi", type=float, default=0.0, help="Ewald parameter (0 = the workload's)")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-sample", type=int, default=-1,
                     help="particles in the CPU reference run (-1 = the whole workload, 0 = skip)")

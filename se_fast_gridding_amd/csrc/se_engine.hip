@@ -112,8 +112,8 @@ struct se_plan {
     // device buffers
     int *key = nullptr, *rank = nullptr, *count = nullptr, *row_start = nullptr, *tile_count = nullptr,
         *nchunk = nullptr, *chunk_start = nullptr, *item_tile = nullptr, *item_beg = nullptr,
-        *item_end = nullptr, *n_items = nullptr, *ncover = nullptr;
-    int2* cover = nullptr;
+        *item_end = nullptr, *n_items = nullptr, *nseg = nullptr;
+    int4* seg = nullptr;
     void *rec = nullptr, *partials = nullptr, *grid = nullptr, *spec = nullptr, *cub_tmp = nullptr,
          *fft_work = nullptr;
     double* gtab = nullptr;
@@ -185,29 +185,46 @@ void derive(const se_plan_desc* d, Geom& g, double& eta, double& m)
     g.d = std::pow(c / M_PI, 1.5);
 }
 
-void build_cover(se_plan* pl)
+// Merge segment tables, one per dimension (the three are identical for a cubic grid):
+// for output tile coordinate t, the runs of sub-grid coordinates l in [0, S) of source
+// tile u whose periodic image (u*TS + l) mod M falls inside [t*TS, t*TS + TS), as
+// (u, first l, first output offset, run length).  A source tile can contribute several
+// runs when its sub-grid wraps around a small grid.
+void build_merge_segments(se_plan* pl)
 {
     const int M = pl->g.M, TS = pl->tl.TS, NT = pl->tl.NT, S = pl->tl.S;
-    std::vector<std::vector<int2>> lists(M);
-    int ncov = 0;
-    for (int g = 0; g < M; ++g) {
-        for (int t = 0; t < NT; ++t) {
-            int l = ((g - t * TS) % M + M) % M;
-            for (; l < S; l += M) lists[g].push_back(make_int2(t, l));
+    std::vector<std::vector<int4>> lists(NT);
+    int nseg = 0;
+    for (int t = 0; t < NT; ++t) {
+        const int o0 = t * TS, len = std::min(TS, M - o0);
+        for (int u = 0; u < NT; ++u) {
+            int run_l = -1, run_o = -1, run_n = 0;
+            for (int l = 0; l < S; ++l) {
+                const int o = (u * TS + l) % M - o0;
+                const bool in = o >= 0 && o < len;
+                if (in && run_n > 0 && o == run_o + run_n && l == run_l + run_n) {
+                    ++run_n;
+                    continue;
+                }
+                if (run_n > 0) lists[t].push_back(make_int4(u, run_l, run_o, run_n));
+                run_n = 0;
+                if (in) { run_l = l; run_o = o; run_n = 1; }
+            }
+            if (run_n > 0) lists[t].push_back(make_int4(u, run_l, run_o, run_n));
         }
-        ncov = std::max<int>(ncov, (int)lists[g].size());
+        nseg = std::max<int>(nseg, (int)lists[t].size());
     }
-    pl->tl.ncov = std::max(ncov, 1);
-    std::vector<int2> flat((size_t)M * pl->tl.ncov, make_int2(0, 0));
-    std::vector<int> cnt(M);
-    for (int g = 0; g < M; ++g) {
-        cnt[g] = (int)lists[g].size();
-        for (size_t a = 0; a < lists[g].size(); ++a) flat[(size_t)g * pl->tl.ncov + a] = lists[g][a];
+    pl->tl.nseg = std::max(nseg, 1);
+    std::vector<int4> flat((size_t)NT * pl->tl.nseg, make_int4(0, 0, 0, 0));
+    std::vector<int> cnt(NT);
+    for (int t = 0; t < NT; ++t) {
+        cnt[t] = (int)lists[t].size();
+        for (size_t a = 0; a < lists[t].size(); ++a) flat[(size_t)t * pl->tl.nseg + a] = lists[t][a];
     }
-    pl->cover = (int2*)pl->alloc(flat.size() * sizeof(int2));
-    pl->ncover = (int*)pl->alloc(M * sizeof(int));
-    SE_HIP(hipMemcpy(pl->cover, flat.data(), flat.size() * sizeof(int2), hipMemcpyHostToDevice));
-    SE_HIP(hipMemcpy(pl->ncover, cnt.data(), M * sizeof(int), hipMemcpyHostToDevice));
+    pl->seg = (int4*)pl->alloc(flat.size() * sizeof(int4));
+    pl->nseg = (int*)pl->alloc(NT * sizeof(int));
+    SE_HIP(hipMemcpy(pl->seg, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
+    SE_HIP(hipMemcpy(pl->nseg, cnt.data(), NT * sizeof(int), hipMemcpyHostToDevice));
 }
 
 size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max)
@@ -268,7 +285,8 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q)
         launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
                                pl->g, tl, (T*)pl->partials, s);
     pl->rec_ev(2);
-    launch_merge<T>((const T*)pl->partials, pl->chunk_start, pl->cover, pl->ncover, pl->g, tl, (T*)pl->grid, s);
+    launch_merge<T>((const T*)pl->partials, pl->chunk_start, pl->seg, pl->nseg, tl.nseg, pl->g, tl, (T*)pl->grid,
+                    s);
     pl->rec_ev(3);
     SE_HIP(hipGetLastError());
 }
@@ -360,7 +378,7 @@ int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
         SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->count, pl->row_start, tl.nbins + 1,
                                                 pl->stream));
         pl->cub_tmp = pl->alloc(pl->cub_bytes);
-        build_cover(pl);
+        build_merge_segments(pl);
         // Green's tables: s = -(1-eta)/(4 xi^2)   (spectral_ewald.c:126)
         launch_green_tables(pl->gtab, d->M, d->box, -(1.0 - pl->eta) / (4.0 * d->xi * d->xi), pl->stream);
         // rocFFT R2C / C2R, lengths fastest-first (z, y, x)

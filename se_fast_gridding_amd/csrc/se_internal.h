@@ -59,7 +59,7 @@ struct Tiling {
     int sub_elems;    // S*S*Szp
     int part_elems;   // S*S*S
     int max_items;    // ntiles + n_cap / chunk + 1
-    int ncov;         // max covering (tile, offset) pairs per coordinate
+    int nseg;         // max merge segments per tile coordinate (segment-table stride)
     int pencil;       // 1: register-ring pencil kernels (FP32, P in the compiled set)
 };
 
@@ -109,8 +109,8 @@ void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* ite
                          const int* item_end, const int* n_items, const Geom& g,
                          const Tiling& tl, T* partials, hipStream_t s);
 template <typename T>
-void launch_merge(const T* partials, const int* chunk_start, const int2* cover,
-                  const int* ncover, const Geom& g, const Tiling& tl, T* grid, hipStream_t s);
+void launch_merge(const T* partials, const int* chunk_start, const int4* seg, const int* nseg, int sstride,
+                  const Geom& g, const Tiling& tl, T* grid, hipStream_t s);
 template <typename T>
 void launch_interp_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const T* grid, const Geom& g,

@@ -329,10 +329,15 @@ void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* ite
 }
 
 // ================================================================================
-// Halo merge (= wrap): each periodic grid point sums, in a fixed order, the
-// partial values of every (tile, chunk) whose sub-grid covers it.  FP32 partials
+// Halo merge (= wrap), owner-tile form: one work-group per tile of the periodic grid;
+// each thread owns up to EPT points of that tile and sums, in a fixed order, every
+// partial value that lands on them: for each box of partial sub-grid elements that
+// maps onto the tile (the product of one per-dimension segment from each of the three
+// segment tables) and each work item of the box's source tile.  Control flow and the
+// tables are uniform across the work-group, partial reads are z-contiguous, and the
+// per-point order is fixed, so the result does not depend on scheduling.  FP32 partials
 // are summed in FP64; FP64 partials with Kahan compensation.
-// cover[g*ncov + a] = (tile coordinate, local offset) pairs covering coordinate g.
+// seg[t * sstride + a] = (source tile coordinate, local start, output start, length).
 // ================================================================================
 template <typename T>
 struct MergeAcc;
@@ -355,43 +360,81 @@ struct MergeAcc<double> {
     __device__ double get() const { return s; }
 };
 
-template <typename T>
-__global__ void k_merge(const T* __restrict__ partials, const int* __restrict__ chunk_start,
-                        const int2* __restrict__ cover, const int* __restrict__ ncover, Geom g, Tiling tl,
-                        T* __restrict__ grid)
+template <typename T, int EPT>
+__global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ partials,
+                                                     const int* __restrict__ chunk_start,
+                                                     const int4* __restrict__ seg, const int* __restrict__ nseg,
+                                                     int sstride, Geom g, Tiling tl, T* __restrict__ grid)
 {
-    const long M = g.M;
-    long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (gid >= M * M * M) return;
-    const int gz = (int)(gid % M);
-    const long r = gid / M;
-    const int gy = (int)(r % M), gx = (int)(r / M);
-    const int S = tl.S, NT = tl.NT, nc = tl.ncov;
-    const int cx = ncover[gx], cy = ncover[gy], cz = ncover[gz];
-    MergeAcc<T> acc;
-    for (int a = 0; a < cx; ++a) {
-        const int2 A = cover[gx * nc + a];
-        for (int b = 0; b < cy; ++b) {
-            const int2 B = cover[gy * nc + b];
-            for (int e = 0; e < cz; ++e) {
-                const int2 C = cover[gz * nc + e];
-                const int tile = (A.x * NT + B.x) * NT + C.x;
-                const int loc = (A.y * S + B.y) * S + C.y;
-                const int i1 = chunk_start[tile + 1];
-                for (int it = chunk_start[tile]; it < i1; ++it) acc.add(partials[(size_t)it * tl.part_elems + loc]);
+    // XCD-aware order: work-groups are dealt to the 8 XCDs round-robin, so XCD
+    // (blockIdx % 8) walks whole z-columns of tiles (tz = 0, 1, ...) of the (tx, ty)
+    // columns congruent to it mod 8.  Neighbours in z read the two ends of the same
+    // z-rows of a partial, which then stay in that XCD's L2.
+    const int NT = tl.NT, TS = tl.TS, S = tl.S, M = g.M;
+    const int xcd = blockIdx.x % 8, k = blockIdx.x / 8;
+    const int col = xcd + 8 * (k / NT), tz = k % NT;
+    if (col >= NT * NT) return;   // uniform: the last round of columns is partial
+    const int tx = col / NT, ty = col % NT;
+    const int ox = tx * TS, oy = ty * TS, oz = tz * TS;
+    const int nx = min(TS, M - ox), ny = min(TS, M - oy), nz = min(TS, M - oz);
+    const int V = nx * ny * nz;
+    int ex[EPT], ey[EPT], ez[EPT];
+    bool own[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int e = threadIdx.x + 256 * k;
+        own[k] = e < V;
+        ez[k] = e % nz;
+        ey[k] = (e / nz) % ny;
+        ex[k] = e / (nz * ny);
+    }
+    MergeAcc<T> acc[EPT];
+    const int na = nseg[tx], nb = nseg[ty], nc = nseg[tz];
+    for (int a = 0; a < na; ++a) {
+        const int4 A = seg[tx * sstride + a];
+        for (int b = 0; b < nb; ++b) {
+            const int4 B = seg[ty * sstride + b];
+            for (int c = 0; c < nc; ++c) {
+                const int4 C = seg[tz * sstride + c];
+                const int src = (A.x * NT + B.x) * NT + C.x;
+                const int i1 = chunk_start[src + 1];
+                for (int it = chunk_start[src]; it < i1; ++it) {
+                    const T* base = partials + (size_t)it * tl.part_elems;
+#pragma unroll
+                    for (int k = 0; k < EPT; ++k) {
+                        const unsigned dx = (unsigned)(ex[k] - A.z), dy = (unsigned)(ey[k] - B.z),
+                                       dz = (unsigned)(ez[k] - C.z);
+                        if (own[k] && dx < (unsigned)A.w && dy < (unsigned)B.w && dz < (unsigned)C.w)
+                            acc[k].add(base[((A.y + (int)dx) * S + (B.y + (int)dy)) * S + (C.y + (int)dz)]);
+                    }
+                }
             }
         }
     }
-    grid[gid] = acc.get();
+#pragma unroll
+    for (int k = 0; k < EPT; ++k)
+        if (own[k]) grid[((long)(ox + ex[k]) * M + (oy + ey[k])) * M + (oz + ez[k])] = acc[k].get();
 }
 
 template <typename T>
-void launch_merge(const T* partials, const int* chunk_start, const int2* cover, const int* ncover, const Geom& g,
-                  const Tiling& tl, T* grid, hipStream_t s)
+void launch_merge(const T* partials, const int* chunk_start, const int4* seg, const int* nseg, int sstride,
+                  const Geom& g, const Tiling& tl, T* grid, hipStream_t s)
 {
-    long n = (long)g.M * g.M * g.M;
-    hipLaunchKernelGGL(k_merge<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, partials, chunk_start, cover, ncover, g,
-                       tl, grid);
+    const int V = tl.TS * tl.TS * tl.TS;
+    const int cols8 = (tl.NT * tl.NT + 7) / 8;
+    dim3 gr(8 * cols8 * tl.NT), bl(256);
+#define SE_MG(E) hipLaunchKernelGGL((k_merge_tiles<T, E>), gr, bl, 0, s, partials, chunk_start, seg, nseg, sstride, g, tl, grid)
+    if (V <= 256)
+        SE_MG(1);
+    else if (V <= 512)
+        SE_MG(2);
+    else if (V <= 1024)
+        SE_MG(4);
+    else if (V <= 2048)
+        SE_MG(8);
+    else
+        SE_MG(16);
+#undef SE_MG
 }
 
 // ================================================================================
@@ -1012,8 +1055,8 @@ void launch_reorder_gather(const double* xin, const double* qin, const int* perm
                                     const int*, const int*, Rec<T>*, hipStream_t);                               \
     template void launch_spread_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \
                                          const Geom&, const Tiling&, T*, hipStream_t);                           \
-    template void launch_merge<T>(const T*, const int*, const int2*, const int*, const Geom&, const Tiling&, T*, \
-                                  hipStream_t);                                                                  \
+    template void launch_merge<T>(const T*, const int*, const int4*, const int*, int, const Geom&, const Tiling&, \
+                                  T*, hipStream_t);                                                              \
     template void launch_interp_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \
                                          const T*, const Geom&, const Tiling&, int, double*, double*,            \
                                          hipStream_t);                                                           \

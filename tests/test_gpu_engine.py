@@ -197,6 +197,40 @@ def test_pencil_kernels_match_oracle_and_generic(oracle, P):
     assert rms_rel(p1.cpu().numpy(), ref_phi) < FP32_TOL and rms_rel(f1.cpu().numpy(), ref_F) < FP32_TOL
 
 
+def clustered_system(N, seed=3, n_clusters=64, frac=0.8, sigma=0.01):
+    """80 % of N in 64 Gaussian clusters (sigma = 0.01 L), 20 % uniform, q = +-1, zero net charge."""
+    rng = np.random.default_rng(seed)
+    nc = int(frac * N)
+    centres = rng.random((3, n_clusters))
+    lab = rng.integers(0, n_clusters, nc)
+    xc = (centres[:, lab] + sigma * rng.standard_normal((3, nc))) % 1.0
+    x = np.concatenate([xc, rng.random((3, N - nc))], axis=1)
+    x = np.where(x >= 1.0, 0.0, x)
+    q = np.ones(N)
+    q[N // 2:] = -1.0
+    rng.shuffle(q)
+    return np.ascontiguousarray(x.reshape(-1)), q
+
+
+def test_clustered_full_size_fp32_matches_fp64():
+    """The clustered workload at full size (N = 4M, M = 256, P = 16, xi = 80): heavy tiles
+    are split into many work items, nothing is dropped, and the FP32 path agrees with the
+    FP64 path at the same P to the FP32 tolerance."""
+    N, M, P, xi = 4_000_000, 256, 16, 80.0
+    x, q = clustered_system(N)
+    print("case clustered N=%d M=%d P=%d" % (N, M, P), flush=True)
+    phi32, F32, info32 = solve(x, q, M, P, xi, se.PREC_FP32)
+    assert info32["pencil"] == 1 and info32["discarded"] == 0
+    # every tile holds uniform particles; the clusters' heavy tiles add items beyond one per tile
+    assert info32["items"] > info32["ntiles"] + 500
+    phi64, F64, info64 = solve(x, q, M, P, xi, se.PREC_FP64)
+    assert info64["discarded"] == 0
+    e_phi, e_F = rms_rel(phi32, phi64), rms_rel(F32, F64)
+    print("clustered fp32 vs fp64: phi %.2e F %.2e, items %d / tiles %d" % (e_phi, e_F, info32["items"],
+                                                                          info32["ntiles"]))
+    assert e_phi < FP32_TOL and e_F < FP32_TOL
+
+
 def test_capacity_is_enforced():
     plan = se.Plan(16, 8, 1.0, 4.0, 10, precision=se.PREC_FP64)
     x = torch.zeros(3 * 11, dtype=torch.float64, device=DEV)
