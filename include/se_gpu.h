@@ -39,10 +39,10 @@ extern "C" {
 typedef struct se_plan se_plan;
 
 typedef struct {
-    int periodicity;   /* 3 (SE3P); 2P/1P gridding primitives are in se_fgg.h */
-    int M;             /* grid points per side of the cubic periodic grid */
+    int periodicity;   /* 3 (SE3P) or 2 (SE2P: periodic in x and y, free in z) */
+    int M;             /* grid points per side of the periodic grid (x and y; also z for SE3P) */
     int P;             /* Gaussian support, 1..32 */
-    double box;        /* side length L of the cubic box */
+    double box;        /* side length L of the box (SE3P: cube; SE2P: the periodic square) */
     double xi;         /* Ewald parameter */
     double m;          /* Gaussian shape; 0 selects the driver's 0.95*sqrt(pi*P) */
     int precision;     /* SE_PREC_FP64 or SE_PREC_FP32 (spread, FFT and gather arithmetic) */
@@ -51,6 +51,10 @@ typedef struct {
     int chunk;         /* max particles per work item, 0 = automatic */
     int block;         /* threads per work-group for spread/gather, 0 = automatic */
     int flags;         /* SE_PLAN_* bits */
+    double slab;       /* SE2P: particles have 0 <= z < slab (ignored for SE3P) */
+    int Mz;            /* SE2P: z grid points of the zero-padded free direction, 0 = automatic
+                          (the smallest FFT-friendly size that holds the slab, the support and
+                          the kernel truncation radius); ignored for SE3P */
 } se_plan_desc;
 
 /* Use the generic one-particle-per-wave kernels even where the FP32 register-ring
@@ -65,7 +69,20 @@ size_t se_plan_bytes(const se_plan* plan);
 /* Bytes a plan with this description would hold, without creating it. */
 size_t se_plan_bytes_for(const se_plan_desc* desc);
 
-/* Full SE3P k-space solve.  phi and/or force may be NULL (not computed). */
+/*
+ * SE2P: the potential and force are the SE2P Ewald k-space sum (SE_direct.c:256-338:
+ * the k_par != 0 terms plus the k_par = 0 term).  The free direction is zero-padded to
+ * Mz points and uses the Coulomb kernel truncated at a radius that covers the slab and
+ * the Gaussian supports, which the padded FFT convolves exactly; z grid index 0 sits
+ * at z0 = a + h/2 with a = -(P_half + 1) h (the reference's SE2P grid origin
+ * convention, SE_fgg.c:753-764).  Particles with z outside [0, slab) are left out and
+ * counted as discarded.
+ */
+
+/* Grid shape of the plan: dims = {M, M, Mz}; *z0 = z coordinate of grid index 0 in z. */
+int se_plan_grid_shape(const se_plan* plan, int dims[3], double* z0);
+
+/* Full k-space solve.  phi and/or force may be NULL (not computed). */
 int se_solve(se_plan* plan, int N, const double* d_x, const double* d_q, double* d_phi, double* d_force);
 
 /* Stages of se_solve, for timing and for callers that bring their own grid. */
@@ -73,7 +90,7 @@ int se_spread(se_plan* plan, int N, const double* d_x, const double* d_q);
 int se_kspace(se_plan* plan);
 int se_interp(se_plan* plan, double* d_phi, double* d_force);
 
-/* Periodic grid of the plan (row-major, z fastest; float or double per precision). */
+/* Grid of the plan (row-major [x][y][z], z fastest, M x M x Mz; float or double per precision). */
 int se_plan_grid(se_plan* plan, void** d_grid, int* elem_bytes);
 
 /*
@@ -89,8 +106,8 @@ int se_plan_timings(se_plan* plan, float ms[8]);
 
 /* Tile/work-item geometry chosen for the plan: out[0]=tile, [1]=chunk, [2]=block,
  * [3]=number of tiles, [4]=work items of the last spread (syncs), [5]=particles of the
- * last spread discarded as non-finite, [6]=sub-grid side S, [7]=1 if the pencil
- * kernels run. */
+ * last spread discarded (non-finite, or outside the SE2P slab), [6]=sub-grid side S,
+ * [7]=1 if the pencil kernels run. */
 int se_plan_info(se_plan* plan, int out[8]);
 
 /* Energy E = 1/2 sum phi q (FP64, on the device), into *d_energy (device double). */

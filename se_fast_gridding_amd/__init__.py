@@ -60,7 +60,8 @@ class se_plan_desc(C.Structure):
     _fields_ = [("periodicity", C.c_int), ("M", C.c_int), ("P", C.c_int),
                 ("box", C.c_double), ("xi", C.c_double), ("m", C.c_double),
                 ("precision", C.c_int), ("n_max", C.c_int), ("tile", C.c_int),
-                ("chunk", C.c_int), ("block", C.c_int), ("flags", C.c_int)]
+                ("chunk", C.c_int), ("block", C.c_int), ("flags", C.c_int),
+                ("slab", C.c_double), ("Mz", C.c_int)]
 
 PLAN_GENERIC_KERNELS = 1
 
@@ -132,6 +133,7 @@ _SIG = {
     "se_kspace": (C.c_int, [_VP]),
     "se_interp": (C.c_int, [_VP, _VP, _VP]),
     "se_plan_grid": (C.c_int, [_VP, C.POINTER(_VP), _I]),
+    "se_plan_grid_shape": (C.c_int, [_VP, _I, _D]),
     "se_plan_enable_timing": (C.c_int, [_VP, C.c_int]),
     "se_plan_timings": (C.c_int, [_VP, C.POINTER(C.c_float)]),
     "se_plan_info": (C.c_int, [_VP, _I]),
@@ -200,23 +202,30 @@ def _ptr(t):
 
 
 class Plan:
-    """Device-resident SE3P solver (include/se_gpu.h) driven with torch CUDA tensors.
+    """Device-resident SE3P / SE2P solver (include/se_gpu.h) driven with torch CUDA tensors.
 
     x is (3N,) float64 SoA [x..., y..., z...], q is (N,) float64, outputs are float64,
-    force SoA (3N,) in the reference's (q/2) grad(phi) convention (SE_fgg.c:2291-2307)."""
+    force SoA (3N,) in the reference's (q/2) grad(phi) convention (SE_fgg.c:2291-2307).
+    periodicity=2 solves SE2P: periodic in x and y (period box), free in z with particles
+    in 0 <= z < slab; Mz (0 = automatic) is the zero-padded z grid."""
 
     def __init__(self, M, P, box, xi, n_max, precision=PREC_FP64, m=0.0, tile=0, chunk=0, block=0,
-                 stream=None, generic=False):
+                 stream=None, generic=False, periodicity=3, slab=0.0, Mz=0):
         import torch
         self._torch = torch
-        d = se_plan_desc(3, M, P, float(box), float(xi), float(m), precision, int(n_max), tile, chunk, block,
-                         PLAN_GENERIC_KERNELS if generic else 0)
+        d = se_plan_desc(periodicity, M, P, float(box), float(xi), float(m), precision, int(n_max), tile, chunk,
+                         block, PLAN_GENERIC_KERNELS if generic else 0, float(slab), int(Mz))
         self.desc = d
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         h = C.c_void_p()
         check(capi().se_plan_create(C.byref(h), C.byref(d), C.c_void_p(self.stream.cuda_stream)), "se_plan_create")
         self.h = h
         self.M, self.P, self.n_max, self.precision = M, P, n_max, precision
+        dims = (C.c_int * 3)()
+        z0 = C.c_double()
+        check(capi().se_plan_grid_shape(h, dims, C.byref(z0)), "se_plan_grid_shape")
+        self.shape = (dims[0], dims[1], dims[2])
+        self.z0 = z0.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -246,10 +255,10 @@ class Plan:
         return p.value, dt, es.value
 
     def grid(self):
-        """Copy of the plan's periodic grid, (M, M, M) row-major (z fastest)."""
+        """Copy of the plan's grid, (M, M, Mz) row-major (z fastest)."""
         torch = self._torch
         ptr, dt, es = self._grid_ptr()
-        out = torch.empty((self.M, self.M, self.M), dtype=dt, device="cuda")
+        out = torch.empty(self.shape, dtype=dt, device="cuda")
         torch.cuda.synchronize()
         _hip_copy(out.data_ptr(), ptr, out.numel() * es)
         return out
@@ -259,7 +268,7 @@ class Plan:
         torch = self._torch
         ptr, dt, es = self._grid_ptr()
         src = g.to(dtype=dt).contiguous()
-        assert src.numel() == self.M ** 3
+        assert src.numel() == self.shape[0] * self.shape[1] * self.shape[2]
         torch.cuda.synchronize()
         _hip_copy(ptr, src.data_ptr(), src.numel() * es)
 

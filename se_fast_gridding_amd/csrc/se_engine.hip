@@ -55,8 +55,8 @@ struct DevMem {
 // Tiling rule.  Spread sub-grids accumulate in FP64 LDS (S^3 * 8 bytes), gather
 // sub-grids hold the grid precision.  Default tile side: the largest (<= 16; even
 // and <= 12 for the pencil kernels) whose FP64 sub-grid fits in LDS.
-static Tiling choose_tiling(int esize, int P, int M, int n_cap, int tile, int chunk, int block, int lds_max,
-                            bool allow_pencil)
+static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int tile, int chunk, int block,
+                            int lds_max, bool allow_pencil)
 {
     Tiling tl{};
     tl.pencil = (allow_pencil && esize == 4 && pencil_supported(P)) ? 1 : 0;
@@ -70,19 +70,21 @@ static Tiling choose_tiling(int esize, int P, int M, int n_cap, int tile, int ch
         }
         SE_REQUIRE(TS > 0, SE_ERR_ARG, "support P too large for the LDS sub-grid");
     }
-    TS = std::min(TS, M);
+    const int Mmin = std::min(M, Mz);
+    TS = std::min(TS, Mmin);
     if (tl.pencil && (TS & 1)) TS -= 1;   // pencils pair columns: even tile side
     if (tl.pencil && TS < 2) {
         tl.pencil = 0;
-        TS = std::max(1, std::min(tile > 0 ? tile : M, M));
+        TS = std::max(1, std::min(tile > 0 ? tile : Mmin, Mmin));
     }
     SE_REQUIRE(TS >= 1 && TS <= 255, SE_ERR_ARG, "tile side out of range");
     tl.TS = TS;
     tl.NT = (M + TS - 1) / TS;
+    tl.NTz = (Mz + TS - 1) / TS;
     tl.S = TS + P - 1;
     tl.Szp = tl.S;
     SE_REQUIRE((long)tl.S * tl.S * tl.Szp * 8 <= lds_max, SE_ERR_ARG, "tile sub-grid exceeds LDS");
-    tl.ntiles = tl.NT * tl.NT * tl.NT;
+    tl.ntiles = tl.NT * tl.NT * tl.NTz;
     tl.PX = (TS + 1) / 2;
     tl.K = tl.PX * tl.PX * TS;
     SE_REQUIRE((long)tl.ntiles * tl.K < (1l << 30), SE_ERR_ARG, "grid too large for the sort keys");
@@ -106,14 +108,15 @@ struct se_plan {
     Geom g{};
     Tiling tl{};
     double L = 1, xi = 1, m = 1, eta = 0;
+    double R2p = 0;   // SE2P kernel truncation radius
     hipStream_t stream = nullptr;
     int n_cap = 0;
     int last_N = -1;
     // device buffers
     int *key = nullptr, *rank = nullptr, *count = nullptr, *row_start = nullptr, *tile_count = nullptr,
         *nchunk = nullptr, *chunk_start = nullptr, *item_tile = nullptr, *item_beg = nullptr,
-        *item_end = nullptr, *n_items = nullptr, *nseg = nullptr;
-    int4* seg = nullptr;
+        *item_end = nullptr, *n_items = nullptr, *nseg = nullptr, *nsegz = nullptr;
+    int4 *seg = nullptr, *segz = nullptr;
     void *rec = nullptr, *partials = nullptr, *grid = nullptr, *spec = nullptr, *cub_tmp = nullptr,
          *fft_work = nullptr;
     double* gtab = nullptr;
@@ -165,8 +168,21 @@ int guarded(F&& f)
     }
 }
 
-void derive(const se_plan_desc* d, Geom& g, double& eta, double& m)
+// Smallest n >= nmin whose prime factors are 2, 3, 5 and 7 (sizes rocFFT handles directly).
+int fft_size(int nmin)
 {
+    for (int n = std::max(nmin, 1);; ++n) {
+        int r = n;
+        for (int f : {2, 3, 5, 7})
+            while (r % f == 0) r /= f;
+        if (r == 1) return n;
+    }
+}
+
+void derive(const se_plan_desc* d, Geom& g, double& eta, double& m, double& R2p)
+{
+    SE_REQUIRE(d->periodicity == 3 || d->periodicity == 2, SE_ERR_ARG,
+               "se_plan: periodicity must be 3 (SE3P) or 2 (SE2P)");
     SE_REQUIRE(d->M >= 1 && d->P >= 1 && d->P <= 32 && d->box > 0 && d->xi > 0, SE_ERR_ARG,
                "invalid plan description (need M>=1, 1<=P<=32, box>0, xi>0)");
     m = d->m > 0 ? d->m : 0.95 * std::sqrt(M_PI * d->P);
@@ -183,6 +199,28 @@ void derive(const se_plan_desc* d, Geom& g, double& eta, double& m)
     g.h = h;
     g.c = c;
     g.d = std::pow(c / M_PI, 1.5);
+    g.per = d->periodicity;
+    if (g.per == 3) {
+        g.Mz = d->M;
+        g.z0 = 0.0;
+        g.zlo = -HUGE_VAL;
+        g.zhi = HUGE_VAL;
+        R2p = 0.0;
+        return;
+    }
+    // SE2P: z grid origin a + h/2 with a = -(P_half + 1) h keeps every support window
+    // of a particle with 0 <= z < slab at index >= 0; the kernel truncation radius R
+    // covers the slab, the support and the Gaussian screening tail, and the padded z
+    // period holds R plus the span D of grid points that carry charge or are read back.
+    SE_REQUIRE(d->slab > 0, SE_ERR_ARG, "SE2P plan needs slab > 0");
+    g.z0 = -(double)(ph + 1) * h + 0.5 * h;
+    g.zlo = 0.0;
+    g.zhi = d->slab;
+    R2p = d->slab + d->P * h + 6.0 / d->xi;
+    const double D = d->slab + (double)(d->P + 2 * ph + 2) * h;
+    const int nmin = (int)std::ceil((R2p + D) / h) + 1;
+    g.Mz = d->Mz > 0 ? d->Mz : fft_size(nmin);
+    SE_REQUIRE(g.Mz >= nmin, SE_ERR_ARG, "SE2P z grid (Mz) too small for the slab, support and kernel radius");
 }
 
 // Merge segment tables, one per dimension (the three are identical for a cubic grid):
@@ -190,11 +228,9 @@ void derive(const se_plan_desc* d, Geom& g, double& eta, double& m)
 // tile u whose periodic image (u*TS + l) mod M falls inside [t*TS, t*TS + TS), as
 // (u, first l, first output offset, run length).  A source tile can contribute several
 // runs when its sub-grid wraps around a small grid.
-void build_merge_segments(se_plan* pl)
+void merge_segments(int M, int TS, int NT, int S, std::vector<std::vector<int4>>& lists)
 {
-    const int M = pl->g.M, TS = pl->tl.TS, NT = pl->tl.NT, S = pl->tl.S;
-    std::vector<std::vector<int4>> lists(NT);
-    int nseg = 0;
+    lists.assign(NT, {});
     for (int t = 0; t < NT; ++t) {
         const int o0 = t * TS, len = std::min(TS, M - o0);
         for (int u = 0; u < NT; ++u) {
@@ -208,34 +244,53 @@ void build_merge_segments(se_plan* pl)
                 }
                 if (run_n > 0) lists[t].push_back(make_int4(u, run_l, run_o, run_n));
                 run_n = 0;
-                if (in) { run_l = l; run_o = o; run_n = 1; }
+                if (in) {
+                    run_l = l;
+                    run_o = o;
+                    run_n = 1;
+                }
             }
             if (run_n > 0) lists[t].push_back(make_int4(u, run_l, run_o, run_n));
         }
-        nseg = std::max<int>(nseg, (int)lists[t].size());
     }
-    pl->tl.nseg = std::max(nseg, 1);
-    std::vector<int4> flat((size_t)NT * pl->tl.nseg, make_int4(0, 0, 0, 0));
-    std::vector<int> cnt(NT);
-    for (int t = 0; t < NT; ++t) {
-        cnt[t] = (int)lists[t].size();
-        for (size_t a = 0; a < lists[t].size(); ++a) flat[(size_t)t * pl->tl.nseg + a] = lists[t][a];
-    }
-    pl->seg = (int4*)pl->alloc(flat.size() * sizeof(int4));
-    pl->nseg = (int*)pl->alloc(NT * sizeof(int));
-    SE_HIP(hipMemcpy(pl->seg, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
-    SE_HIP(hipMemcpy(pl->nseg, cnt.data(), NT * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void build_merge_segments(se_plan* pl)
+{
+    const int TS = pl->tl.TS, S = pl->tl.S;
+    std::vector<std::vector<int4>> lx, lz;
+    merge_segments(pl->g.M, TS, pl->tl.NT, S, lx);
+    merge_segments(pl->g.Mz, TS, pl->tl.NTz, S, lz);
+    int nseg = 1;
+    for (auto& v : lx) nseg = std::max<int>(nseg, (int)v.size());
+    for (auto& v : lz) nseg = std::max<int>(nseg, (int)v.size());
+    pl->tl.nseg = nseg;
+    auto upload = [&](const std::vector<std::vector<int4>>& lists, int4*& seg, int*& cnt) {
+        std::vector<int4> flat(lists.size() * nseg, make_int4(0, 0, 0, 0));
+        std::vector<int> c(lists.size());
+        for (size_t t = 0; t < lists.size(); ++t) {
+            c[t] = (int)lists[t].size();
+            for (size_t a = 0; a < lists[t].size(); ++a) flat[t * nseg + a] = lists[t][a];
+        }
+        seg = (int4*)pl->alloc(flat.size() * sizeof(int4));
+        cnt = (int*)pl->alloc(c.size() * sizeof(int));
+        SE_HIP(hipMemcpy(seg, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
+        SE_HIP(hipMemcpy(cnt, c.data(), c.size() * sizeof(int), hipMemcpyHostToDevice));
+    };
+    upload(lx, pl->seg, pl->nseg);
+    upload(lz, pl->segz, pl->nsegz);
 }
 
 size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max)
 {
     Geom g;
     double eta, m;
-    derive(d, g, eta, m);
+    double R2p = 0;
+    derive(d, g, eta, m, R2p);
     const int es = d->precision == SE_PREC_FP32 ? 4 : 8;
-    Tiling tl = choose_tiling(es, d->P, d->M, d->n_max, d->tile, d->chunk, d->block, lds_max,
+    Tiling tl = choose_tiling(es, d->P, d->M, g.Mz, d->n_max, d->tile, d->chunk, d->block, lds_max,
                               !(d->flags & SE_PLAN_GENERIC_KERNELS));
-    const size_t M = d->M, N = d->n_max;
+    const size_t M = d->M, Mz = g.Mz, N = d->n_max;
     size_t b = 0;
     b += 2 * N * 4;                                   // key, rank
     b += 2 * (size_t)(tl.nbins + 1) * 4;              // count, row_start
@@ -243,9 +298,10 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max)
     b += 3 * (size_t)tl.max_items * 4;                // items
     b += N * (es == 4 ? 32 : 48);                     // records
     b += (size_t)tl.max_items * tl.part_elems * es;   // partials
-    b += M * M * M * es;                              // grid
-    b += M * M * (M / 2 + 1) * 2 * es;                // half spectrum
-    b += M * M * (M / 2 + 1) * 2 * es;                // FFT work (upper estimate)
+    b += M * M * Mz * es;                             // grid
+    b += M * M * (Mz / 2 + 1) * 2 * es;               // half spectrum
+    b += M * M * (Mz / 2 + 1) * 2 * es;               // FFT work (upper estimate)
+    if (g.per == 2) b += M * M * (Mz / 2 + 1) * 8;    // SE2P Green's function table
     return b;
 }
 
@@ -285,8 +341,8 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q)
         launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
                                pl->g, tl, (T*)pl->partials, s);
     pl->rec_ev(2);
-    launch_merge<T>((const T*)pl->partials, pl->chunk_start, pl->seg, pl->nseg, tl.nseg, pl->g, tl, (T*)pl->grid,
-                    s);
+    launch_merge<T>((const T*)pl->partials, pl->chunk_start, pl->seg, pl->nseg, pl->segz, pl->nsegz, tl.nseg, pl->g,
+                    tl, (T*)pl->grid, s);
     pl->rec_ev(3);
     SE_HIP(hipGetLastError());
 }
@@ -315,10 +371,19 @@ void run_kspace(se_plan* pl)
     void* in[1] = {pl->grid};
     void* out[1] = {pl->spec};
     SE_FFT(rocfft_execute(pl->fwd, in, out, pl->info));
-    if (pl->esize == 4)
-        launch_green<float>(pl->spec, pl->g.M, pl->gtab, pl->stream);
-    else
-        launch_green<double>(pl->spec, pl->g.M, pl->gtab, pl->stream);
+    const int M = pl->g.M, Mz = pl->g.Mz;
+    if (pl->g.per == 3) {
+        if (pl->esize == 4)
+            launch_green<float>(pl->spec, M, Mz, pl->gtab, pl->stream);
+        else
+            launch_green<double>(pl->spec, M, Mz, pl->gtab, pl->stream);
+    } else {
+        const long n = (long)M * M * (Mz / 2 + 1);
+        if (pl->esize == 4)
+            launch_green_full<float>(pl->spec, n, pl->gtab, pl->stream);
+        else
+            launch_green_full<double>(pl->spec, n, pl->gtab, pl->stream);
+    }
     void* in2[1] = {pl->spec};
     void* out2[1] = {pl->grid};
     SE_FFT(rocfft_execute(pl->bwd, in2, out2, pl->info));
@@ -345,20 +410,19 @@ int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
     *out = nullptr;
     se_plan* pl = new se_plan();
     int rc = guarded([&] {
-        SE_REQUIRE(d->periodicity == 3, SE_ERR_ARG, "se_plan: only periodicity 3 (SE3P) has a fused solver");
         SE_REQUIRE(d->n_max >= 0, SE_ERR_ARG, "n_max must be >= 0");
         pl->desc = *d;
         pl->esize = d->precision == SE_PREC_FP32 ? 4 : 8;
-        derive(d, pl->g, pl->eta, pl->m);
+        derive(d, pl->g, pl->eta, pl->m, pl->R2p);
         pl->L = d->box;
         pl->xi = d->xi;
         pl->stream = (hipStream_t)stream;
         pl->n_cap = std::max(d->n_max, 1);
         const int lds_max = lds_limit();
-        pl->tl = choose_tiling(pl->esize, d->P, d->M, pl->n_cap, d->tile, d->chunk, d->block, lds_max,
+        pl->tl = choose_tiling(pl->esize, d->P, d->M, pl->g.Mz, pl->n_cap, d->tile, d->chunk, d->block, lds_max,
                                !(d->flags & SE_PLAN_GENERIC_KERNELS));
         const Tiling& tl = pl->tl;
-        const size_t M = d->M, N = pl->n_cap;
+        const size_t M = d->M, Mz = pl->g.Mz, N = pl->n_cap;
         pl->key = (int*)pl->alloc(N * 4);
         pl->rank = (int*)pl->alloc(N * 4);
         pl->count = (int*)pl->alloc(((size_t)tl.nbins + 1) * 4);
@@ -372,18 +436,22 @@ int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
         pl->n_items = (int*)pl->alloc(16);
         pl->rec = pl->alloc(N * (pl->esize == 4 ? sizeof(Rec<float>) : sizeof(Rec<double>)));
         pl->partials = pl->alloc((size_t)tl.max_items * tl.part_elems * pl->esize);
-        pl->grid = pl->alloc(M * M * M * pl->esize);
-        pl->spec = pl->alloc(M * M * (M / 2 + 1) * 2 * pl->esize);
-        pl->gtab = (double*)pl->alloc((2 * M + 1) * sizeof(double));
+        pl->grid = pl->alloc(M * M * Mz * pl->esize);
+        pl->spec = pl->alloc(M * M * (Mz / 2 + 1) * 2 * pl->esize);
+        pl->gtab = (double*)pl->alloc((pl->g.per == 3 ? 2 * M + 2 * Mz + 1 : M * M * (Mz / 2 + 1)) * sizeof(double));
         SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->count, pl->row_start, tl.nbins + 1,
                                                 pl->stream));
         pl->cub_tmp = pl->alloc(pl->cub_bytes);
         build_merge_segments(pl);
-        // Green's tables: s = -(1-eta)/(4 xi^2)   (spectral_ewald.c:126)
-        launch_green_tables(pl->gtab, d->M, d->box, -(1.0 - pl->eta) / (4.0 * d->xi * d->xi), pl->stream);
+        // Green's function: s = -(1-eta)/(4 xi^2)   (spectral_ewald.c:126)
+        const double gs = -(1.0 - pl->eta) / (4.0 * d->xi * d->xi);
+        if (pl->g.per == 3)
+            launch_green_tables(pl->gtab, d->M, (int)Mz, d->box, d->box, gs, pl->stream);
+        else
+            launch_green2p_table(pl->gtab, d->M, (int)Mz, d->box, pl->g.h, pl->R2p, gs, pl->stream);
         // rocFFT R2C / C2R, lengths fastest-first (z, y, x)
         rocfft_init_once();
-        size_t lens[3] = {M, M, M};
+        size_t lens[3] = {Mz, M, M};
         rocfft_precision prec = pl->esize == 4 ? rocfft_precision_single : rocfft_precision_double;
         SE_FFT(rocfft_plan_create(&pl->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, prec, 3,
                                   lens, 1, nullptr));
@@ -469,6 +537,15 @@ int se_solve(se_plan* pl, int N, const double* x, const double* q, double* phi, 
     rc = se_kspace(pl);
     if (rc) return rc;
     return se_interp(pl, phi, force);
+}
+
+int se_plan_grid_shape(const se_plan* pl, int dims[3], double* z0)
+{
+    if (!pl || !dims) return SE_ERR_ARG;
+    dims[0] = dims[1] = pl->g.M;
+    dims[2] = pl->g.Mz;
+    if (z0) *z0 = pl->g.z0;
+    return SE_OK;
 }
 
 int se_plan_grid(se_plan* pl, void** d_grid, int* elem_bytes)

@@ -33,24 +33,33 @@ struct Error {
     } while (0)
 
 // ---------------------------------------------------------------------------------
-// Geometry of one cubic SE3P grid as the fused kernels see it.
-// Support point i of a particle sits at periodic index (iota + first + i) mod M,
-// iota = round(x/h) for odd P, floor(x/h) for even P (SE_fgg.c:554-573).
+// Geometry of one grid as the fused kernels see it: M x M x Mz points, periodic in x
+// and y with period M h, and in z with period Mz h.  Support point i of a particle sits
+// at index (iota + first + i) mod M, iota = round(u) for odd P, floor(u) for even P,
+// u = x/h in x and y and u = (z - z0)/h in z (SE_fgg.c:554-573 for SE3P, where Mz = M
+// and z0 = 0).  For SE2P (per == 2) the z grid is the free direction, zero-padded so
+// that no support window reaches its ends; z0 = a + h/2 is the reference's 2P grid
+// origin (SE_fgg.c:753-764) and particles outside [zlo, zhi) are left out.
 // ---------------------------------------------------------------------------------
 struct Geom {
-    int M;        // grid points per side
+    int M;        // grid points per side in x and y
     int P;        // support
     int first;    // -P_half (odd P) or 1-P_half (even P)
     int odd;
     double h, c, d;   // spacing, Gaussian exponent, normalisation (c/pi)^1.5
+    int Mz;       // grid points in z
+    int per;      // 3: SE3P, 2: SE2P (z free)
+    double z0;    // z coordinate of grid index 0 in z
+    double zlo, zhi;  // SE2P slab: accepted particle z range
 };
 
 // Tile decomposition used by the bucketed spread/gather.  A tile of TS^3 window
 // start points owns an LDS sub-grid of S^3 points, S = TS + P - 1.
 struct Tiling {
-    int TS, NT, S;    // tile side, tiles per side (ceil(M/TS)), sub-grid side
+    int TS, NT, S;    // tile side, tiles per side in x and y (ceil(M/TS)), sub-grid side
+    int NTz;          // tiles in z (ceil(Mz/TS))
     int Szp;          // z pitch of the LDS sub-grid (>= S)
-    int ntiles;       // NT^3
+    int ntiles;       // NT * NT * NTz
     int PX;           // 2x2-column pencils per tile side: ceil(TS/2)
     int K;            // sort keys per tile: PX*PX*TS
     int nbins;        // ntiles*K (the discard bucket is nbins)
@@ -108,16 +117,23 @@ template <typename T>
 void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const Geom& g,
                          const Tiling& tl, T* partials, hipStream_t s);
+// merge segment tables: x and y share (seg, nseg), z has (segz, nsegz); both strides tl.nseg
 template <typename T>
-void launch_merge(const T* partials, const int* chunk_start, const int4* seg, const int* nseg, int sstride,
-                  const Geom& g, const Tiling& tl, T* grid, hipStream_t s);
+void launch_merge(const T* partials, const int* chunk_start, const int4* seg, const int* nseg, const int4* segz,
+                  const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid, hipStream_t s);
 template <typename T>
 void launch_interp_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const T* grid, const Geom& g,
                          const Tiling& tl, int N, double* phi, double* force, hipStream_t s);
+// SE3P Green's function, separable tables: tab = [k_x^2 (M), e_x (M), k_z^2 (Mz), e_z (Mz), 4 pi/(M M Mz)]
 template <typename T>
-void launch_green(void* spec, int M, const double* tab, hipStream_t s);
-void launch_green_tables(double* tab, int M, double L, double scalar, hipStream_t s);
+void launch_green(void* spec, int M, int Mz, const double* tab, hipStream_t s);
+void launch_green_tables(double* tab, int M, int Mz, double L, double Lz, double scalar, hipStream_t s);
+// SE2P Green's function, full half-spectrum table (see se_kernels.hip)
+void launch_green2p_table(double* tab, int M, int Mz, double L, double h, double R, double scalar,
+                          hipStream_t s);
+template <typename T>
+void launch_green_full(void* spec, long n, const double* tab, hipStream_t s);
 void launch_energy(const double* phi, const double* q, int N, double* out, hipStream_t s);
 
 // legacy (FP64, reference layout)

@@ -103,6 +103,18 @@ __device__ __forceinline__ int window_start(double x, const Geom& g, double& t, 
     return imod(iota + g.first, g.M);
 }
 
+// z: offset by the grid origin z0; for SE2P the particle must lie in the slab
+__device__ __forceinline__ int window_start_z(double z, const Geom& g, double& t, bool& ok)
+{
+    const double zr = z - g.z0;
+    double u = zr / g.h;
+    ok = fabs(u) < 1.0e9 && (g.per != 2 || (z >= g.zlo && z < g.zhi));
+    if (!ok) u = 0.0;
+    int iota = g.odd ? (int)round(u) : (int)floor(u);
+    t = zr - g.h * iota;
+    return imod(iota + g.first, g.Mz);
+}
+
 // Counting-sort key of a particle: its window start cell, ordered tile-major, then
 // by 2x2-column pencil (px, pz), then by row y inside the tile:
 //   key = tile * K + ((px * PX + pz) * TS + ly),  K = PX * PX * TS,  PX = ceil(TS/2).
@@ -112,7 +124,7 @@ __device__ __forceinline__ int row_key(int w0, int w1, int w2, const Tiling& tl)
 {
     const int tx = w0 / tl.TS, ty = w1 / tl.TS, tz = w2 / tl.TS;
     const int lx = w0 - tx * tl.TS, ly = w1 - ty * tl.TS, lz = w2 - tz * tl.TS;
-    const int tile = (tx * tl.NT + ty) * tl.NT + tz;
+    const int tile = (tx * tl.NT + ty) * tl.NTz + tz;
     return tile * tl.K + (((lx >> 1) * tl.PX + (lz >> 1)) * tl.TS + ly);
 }
 
@@ -126,8 +138,9 @@ __global__ void k_bin_keys(const double* __restrict__ x, int N, Geom g, Tiling t
     double t;
     int w0 = window_start(x[n], g, t, ok0);
     int w1 = window_start(x[n + (long)N], g, t, ok1);
-    int w2 = window_start(x[n + 2l * N], g, t, ok2);
-    // non-finite or absurd positions go to a discard bucket after all tiles
+    int w2 = window_start_z(x[n + 2l * N], g, t, ok2);
+    // non-finite or absurd positions (and, for SE2P, positions outside the slab) go to a
+    // discard bucket after all tiles
     int k = (ok0 && ok1 && ok2) ? row_key(w0, w1, w2, tl) : tl.nbins;
     key[n] = k;
     rank[n] = atomicAdd(&count[k], 1);
@@ -198,7 +211,7 @@ __global__ void k_scatter(const double* __restrict__ x, const double* __restrict
     double t0, t1, t2;
     int w0 = window_start(x[n], g, t0, ok);
     int w1 = window_start(x[n + (long)N], g, t1, ok);
-    int w2 = window_start(x[n + 2l * N], g, t2, ok);
+    int w2 = window_start_z(x[n + 2l * N], g, t2, ok);
     const int k = key[n];
     Rec<T> r;
     r.t[0] = (T)t0;
@@ -364,19 +377,20 @@ template <typename T, int EPT>
 __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ partials,
                                                      const int* __restrict__ chunk_start,
                                                      const int4* __restrict__ seg, const int* __restrict__ nseg,
+                                                     const int4* __restrict__ segz, const int* __restrict__ nsegz,
                                                      int sstride, Geom g, Tiling tl, T* __restrict__ grid)
 {
     // XCD-aware order: work-groups are dealt to the 8 XCDs round-robin, so XCD
     // (blockIdx % 8) walks whole z-columns of tiles (tz = 0, 1, ...) of the (tx, ty)
     // columns congruent to it mod 8.  Neighbours in z read the two ends of the same
     // z-rows of a partial, which then stay in that XCD's L2.
-    const int NT = tl.NT, TS = tl.TS, S = tl.S, M = g.M;
+    const int NT = tl.NT, NTz = tl.NTz, TS = tl.TS, S = tl.S, M = g.M, Mz = g.Mz;
     const int xcd = blockIdx.x % 8, k = blockIdx.x / 8;
-    const int col = xcd + 8 * (k / NT), tz = k % NT;
+    const int col = xcd + 8 * (k / NTz), tz = k % NTz;
     if (col >= NT * NT) return;   // uniform: the last round of columns is partial
     const int tx = col / NT, ty = col % NT;
     const int ox = tx * TS, oy = ty * TS, oz = tz * TS;
-    const int nx = min(TS, M - ox), ny = min(TS, M - oy), nz = min(TS, M - oz);
+    const int nx = min(TS, M - ox), ny = min(TS, M - oy), nz = min(TS, Mz - oz);
     const int V = nx * ny * nz;
     int ex[EPT], ey[EPT], ez[EPT];
     bool own[EPT];
@@ -389,14 +403,14 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
         ex[k] = e / (nz * ny);
     }
     MergeAcc<T> acc[EPT];
-    const int na = nseg[tx], nb = nseg[ty], nc = nseg[tz];
+    const int na = nseg[tx], nb = nseg[ty], nc = nsegz[tz];
     for (int a = 0; a < na; ++a) {
         const int4 A = seg[tx * sstride + a];
         for (int b = 0; b < nb; ++b) {
             const int4 B = seg[ty * sstride + b];
             for (int c = 0; c < nc; ++c) {
-                const int4 C = seg[tz * sstride + c];
-                const int src = (A.x * NT + B.x) * NT + C.x;
+                const int4 C = segz[tz * sstride + c];
+                const int src = (A.x * NT + B.x) * NTz + C.x;
                 const int i1 = chunk_start[src + 1];
                 for (int it = chunk_start[src]; it < i1; ++it) {
                     const T* base = partials + (size_t)it * tl.part_elems;
@@ -413,17 +427,19 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
     }
 #pragma unroll
     for (int k = 0; k < EPT; ++k)
-        if (own[k]) grid[((long)(ox + ex[k]) * M + (oy + ey[k])) * M + (oz + ez[k])] = acc[k].get();
+        if (own[k]) grid[((long)(ox + ex[k]) * M + (oy + ey[k])) * Mz + (oz + ez[k])] = acc[k].get();
 }
 
 template <typename T>
-void launch_merge(const T* partials, const int* chunk_start, const int4* seg, const int* nseg, int sstride,
-                  const Geom& g, const Tiling& tl, T* grid, hipStream_t s)
+void launch_merge(const T* partials, const int* chunk_start, const int4* seg, const int* nseg, const int4* segz,
+                  const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid, hipStream_t s)
 {
     const int V = tl.TS * tl.TS * tl.TS;
     const int cols8 = (tl.NT * tl.NT + 7) / 8;
-    dim3 gr(8 * cols8 * tl.NT), bl(256);
-#define SE_MG(E) hipLaunchKernelGGL((k_merge_tiles<T, E>), gr, bl, 0, s, partials, chunk_start, seg, nseg, sstride, g, tl, grid)
+    dim3 gr(8 * cols8 * tl.NTz), bl(256);
+#define SE_MG(E)                                                                                              \
+    hipLaunchKernelGGL((k_merge_tiles<T, E>), gr, bl, 0, s, partials, chunk_start, seg, nseg, segz, nsegz,    \
+                       sstride, g, tl, grid)
     if (V <= 256)
         SE_MG(1);
     else if (V <= 512)
@@ -457,14 +473,14 @@ __global__ __launch_bounds__(512) void k_interp_tiles(
     T* sub = reinterpret_cast<T*>(smem_raw);
     const int P = PT > 0 ? PT : g.P;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int M = g.M, S = tl.S, NT = tl.NT;
+    const int M = g.M, Mz = g.Mz, S = tl.S, NT = tl.NT, NTz = tl.NTz;
     {
         const int tile = item_tile[item];
-        const int ox = (tile / (NT * NT)) * tl.TS, oy = ((tile / NT) % NT) * tl.TS, oz = (tile % NT) * tl.TS;
+        const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
         for (int e = tid; e < tl.part_elems; e += nthr) {
             int iz = e % S, r2 = e / S, iy = r2 % S, ix = r2 / S;
-            int gx = (ox + ix) % M, gy = (oy + iy) % M, gz = (oz + iz) % M;
-            sub[(ix * S + iy) * tl.Szp + iz] = grid[((long)gx * M + gy) * M + gz];
+            int gx = (ox + ix) % M, gy = (oy + iy) % M, gz = (oz + iz) % Mz;
+            sub[(ix * S + iy) * tl.Szp + iz] = grid[((long)gx * M + gy) * Mz + gz];
         }
     }
     __syncthreads();
@@ -600,42 +616,115 @@ void launch_interp_tiles(const Rec<T>* rec, const int* item_tile, const int* ite
 // exp(s K^2) = prod_d exp(s k_d^2) from a per-axis table (no M^3 exp).
 // tab[0..M) = k^2, tab[M..2M) = exp(s k^2), tab[2M] = 4pi/M^3.
 // ================================================================================
-__global__ void k_green_tables(double* tab, int M, double L, double scalar)
+// SE3P Green's function (spectral_ewald.c:126-130, SE_general.c:141-176), separable:
+// Z = 4 pi/(M M Mz) e^{scalar k^2} / k^2 with e^{scalar k^2} = e_x e_y e_z, Z(0) = 0.
+// tab = [k_x^2 (M), e_x (M), k_z^2 (Mz), e_z (Mz), 4 pi/(M M Mz)].
+__device__ __forceinline__ double kfreq(int i, int n, double period)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) tab[2 * M] = 4.0 * M_PI / ((double)M * M * M);
-    if (i >= M) return;
-    const int pos = (M % 2 == 0) ? M / 2 : (M - 1) / 2 + 1;
-    double k = 2.0 * M_PI / L * (double)(i < pos ? i : i - M);
-    tab[i] = k * k;
-    tab[M + i] = exp(scalar * k * k);
+    const int pos = (n % 2 == 0) ? n / 2 : (n - 1) / 2 + 1;   // k_vec (SE_general.c:34-84)
+    return 2.0 * M_PI / period * (double)(i < pos ? i : i - n);
 }
 
-void launch_green_tables(double* tab, int M, double L, double scalar, hipStream_t s)
+__global__ void k_green_tables(double* tab, int M, int Mz, double L, double Lz, double scalar)
 {
-    hipLaunchKernelGGL(k_green_tables, dim3(cdiv(M, 256)), dim3(256), 0, s, tab, M, L, scalar);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) tab[2 * M + 2 * Mz] = 4.0 * M_PI / ((double)M * M * Mz);
+    if (i < M) {
+        const double k = kfreq(i, M, L);
+        tab[i] = k * k;
+        tab[M + i] = exp(scalar * k * k);
+    }
+    if (i < Mz) {
+        const double k = kfreq(i, Mz, Lz);
+        tab[2 * M + i] = k * k;
+        tab[2 * M + Mz + i] = exp(scalar * k * k);
+    }
+}
+
+void launch_green_tables(double* tab, int M, int Mz, double L, double Lz, double scalar, hipStream_t s)
+{
+    const int n = M > Mz ? M : Mz;
+    hipLaunchKernelGGL(k_green_tables, dim3(cdiv(n, 256)), dim3(256), 0, s, tab, M, Mz, L, Lz, scalar);
 }
 
 template <typename T>
-__global__ void k_green(T* __restrict__ spec, int M, const double* __restrict__ tab)
+__global__ void k_green(T* __restrict__ spec, int M, int Mz, const double* __restrict__ tab)
 {
-    const int Mh = M / 2 + 1;
+    const int Mh = Mz / 2 + 1;
     long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (gid >= (long)M * M * Mh) return;
     const int kc = (int)(gid % Mh);
     const long r = gid / Mh;
     const int j = (int)(r % M), i = (int)(r / M);
-    const double K2 = (tab[i] + tab[j]) + tab[kc];
-    const double z = (K2 > 0.0) ? tab[M + i] * tab[M + j] * tab[M + kc] / K2 * tab[2 * M] : 0.0;
+    const double* kz2 = tab + 2 * M;
+    const double* ez = tab + 2 * M + Mz;
+    const double K2 = (tab[i] + tab[j]) + kz2[kc];
+    const double z = (K2 > 0.0) ? tab[M + i] * tab[M + j] * ez[kc] / K2 * tab[2 * M + 2 * Mz] : 0.0;
     spec[2 * gid] = (T)((double)spec[2 * gid] * z);
     spec[2 * gid + 1] = (T)((double)spec[2 * gid + 1] * z);
 }
 
 template <typename T>
-void launch_green(void* spec, int M, const double* tab, hipStream_t s)
+void launch_green(void* spec, int M, int Mz, const double* tab, hipStream_t s)
 {
-    long n = (long)M * M * (M / 2 + 1);
-    hipLaunchKernelGGL(k_green<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, (T*)spec, M, tab);
+    long n = (long)M * M * (Mz / 2 + 1);
+    hipLaunchKernelGGL(k_green<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, (T*)spec, M, Mz, tab);
+}
+
+// SE2P Green's function on the half spectrum of the M x M x Mz grid, z zero-padded.
+// The free direction uses the kernel truncated at |z| <= R, whose transform is exact on
+// the padded period Mz h >= R + (slab span), so the periodic FFT convolution equals the
+// free one in z (truncated-kernel method for free-space convolution):
+//   k_par != 0:  FT of (2 pi/k) e^{-k|z|} 1(|z|<=R) = 4 pi [k - e^{-kR}(k cos kR' - kz sin kR')] / (k (k^2+kz^2)),
+//                kR' = kz R (k = |k_par|),
+//   k_par == 0:  FT of -2 pi |z| 1(|z|<=R) = 4 pi [2 sin^2(kz R/2)/kz^2 - R sin(kz R)/kz],  -2 pi R^2 at kz = 0,
+// times e^{scalar (k_par^2 + kz^2)} / (M M Mz), scalar = -(1-eta)/(4 xi^2).  For R -> inf
+// the k_par != 0 part is the SE3P kernel 4 pi/k^2; the potential converges to the SE2P
+// Ewald k-space sum (SE_direct.c:256-338).
+__global__ void k_green2p_table(double* __restrict__ tab, int M, int Mz, double L, double h, double R,
+                                double scalar)
+{
+    const int Mh = Mz / 2 + 1;
+    long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= (long)M * M * Mh) return;
+    const int kc = (int)(gid % Mh);
+    const long r = gid / Mh;
+    const int j = (int)(r % M), i = (int)(r / M);
+    const double kx = kfreq(i, M, L), ky = kfreq(j, M, L);
+    const double kz = 2.0 * M_PI / ((double)Mz * h) * (double)kc;
+    const double kp2 = kx * kx + ky * ky, kp = sqrt(kp2), k2 = kp2 + kz * kz;
+    double G;
+    if (kp > 0.0) {
+        G = 4.0 * M_PI / (kp * k2) * (kp - exp(-kp * R) * (kp * cos(kz * R) - kz * sin(kz * R)));
+    } else if (kc == 0) {
+        G = -2.0 * M_PI * R * R;
+    } else {
+        const double sh = sin(0.5 * kz * R);
+        G = 4.0 * M_PI * (2.0 * sh * sh / (kz * kz) - R * sin(kz * R) / kz);
+    }
+    tab[gid] = G * exp(scalar * k2) / ((double)M * M * Mz);
+}
+
+void launch_green2p_table(double* tab, int M, int Mz, double L, double h, double R, double scalar, hipStream_t s)
+{
+    long n = (long)M * M * (Mz / 2 + 1);
+    hipLaunchKernelGGL(k_green2p_table, dim3(cdiv(n, 256)), dim3(256), 0, s, tab, M, Mz, L, h, R, scalar);
+}
+
+template <typename T>
+__global__ void k_green_full(T* __restrict__ spec, long n, const double* __restrict__ tab)
+{
+    long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    const double z = tab[gid];
+    spec[2 * gid] = (T)((double)spec[2 * gid] * z);
+    spec[2 * gid + 1] = (T)((double)spec[2 * gid + 1] * z);
+}
+
+template <typename T>
+void launch_green_full(void* spec, long n, const double* tab, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_green_full<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, (T*)spec, n, tab);
 }
 
 // E = 1/2 sum phi q  (SE_fgg.c:5934-5946), FP64 block tree + one atomic per block.
@@ -1055,12 +1144,13 @@ void launch_reorder_gather(const double* xin, const double* qin, const int* perm
                                     const int*, const int*, Rec<T>*, hipStream_t);                               \
     template void launch_spread_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \
                                          const Geom&, const Tiling&, T*, hipStream_t);                           \
-    template void launch_merge<T>(const T*, const int*, const int4*, const int*, int, const Geom&, const Tiling&, \
-                                  T*, hipStream_t);                                                              \
+    template void launch_merge<T>(const T*, const int*, const int4*, const int*, const int4*, const int*, int,  \
+                                  const Geom&, const Tiling&, T*, hipStream_t);                                  \
     template void launch_interp_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \
                                          const T*, const Geom&, const Tiling&, int, double*, double*,            \
                                          hipStream_t);                                                           \
-    template void launch_green<T>(void*, int, const double*, hipStream_t);
+    template void launch_green<T>(void*, int, int, const double*, hipStream_t);                                \
+    template void launch_green_full<T>(void*, long, const double*, hipStream_t);
 SE_INST(float)
 SE_INST(double)
 

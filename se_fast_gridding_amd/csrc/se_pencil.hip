@@ -356,15 +356,15 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
     if (item >= *n_items) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
-    const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, NT = tl.NT;
+    const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
     const int tid = threadIdx.x, nthr = blockDim.x;
     {
         const int tile = item_tile[item];
-        const int ox = (tile / (NT * NT)) * tl.TS, oy = ((tile / NT) % NT) * tl.TS, oz = (tile % NT) * tl.TS;
+        const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
         for (int e = tid; e < S3; e += nthr) {
             const int iz = e % S, r2 = e / S, ix = r2 % S, iy = r2 / S;
-            const int gx = (ox + ix) % M, gy = (oy + iy) % M, gz = (oz + iz) % M;
-            sub[e] = grid[((long)gx * M + gy) * M + gz];
+            const int gx = (ox + ix) % M, gy = (oy + iy) % M, gz = (oz + iz) % Mz;
+            sub[e] = grid[((long)gx * M + gy) * Mz + gz];
         }
     }
     __syncthreads();

@@ -105,9 +105,31 @@ def test_host_parameter_routines():
 def test_plan_rejects_bad_descriptions_without_gpu_compute():
     # argument validation happens before any device work
     lib = se.capi()
-    d = se.se_plan_desc(3, 0, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0, 0)
     h = C.c_void_p()
+    d = se.se_plan_desc(3, 0, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0, 0, 0.0, 0)
     assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
-    d = se.se_plan_desc(2, 32, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0, 0)
+    d = se.se_plan_desc(1, 32, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0, 0, 1.0, 0)
     assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
     assert "periodicity" in se.last_error()
+    d = se.se_plan_desc(2, 32, 16, 1.0, 6.0, 0.0, 1, 10, 0, 0, 0, 0, 0.0, 0)   # SE2P without a slab
+    assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
+    assert "slab" in se.last_error()
+    d = se.se_plan_desc(2, 32, 16, 1.0, 6.0, 0.0, 1, 10, 0, 0, 0, 0, 1.0, 40)  # z grid shorter than the slab
+    assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
+    assert "Mz" in se.last_error()
+
+
+def test_plan_desc_layout_matches_header():
+    prog = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "se_gpu.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(se_plan_desc), offsetof(se_plan_desc, box), offsetof(se_plan_desc, flags),
+         offsetof(se_plan_desc, slab), offsetof(se_plan_desc, Mz));
+  return 0;
+}
+"""
+    got = _compile_run(prog, se.INCLUDE_DIR)
+    D = se.se_plan_desc
+    assert [int(v) for v in got] == [C.sizeof(D), D.box.offset, D.flags.offset, D.slab.offset, D.Mz.offset]
