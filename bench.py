@@ -250,13 +250,172 @@ def run(args):
     print(json.dumps(out))
 
 
+def mixed_systems(n_systems, seed=4, se2p_frac=0.0):
+    """The batched workload: N log-uniform in [1e3, 1e5], M in {32, 48, 64, 96}, xi = 0.3125 M,
+    P in {11, 16}, x uniform, q = +-1 with zero net charge; a fraction of the systems SE2P
+    (unit square periodic, slab height 1)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_systems):
+        N = int(np.floor(np.exp(rng.uniform(np.log(1e3), np.log(1e5)))))
+        M = int(rng.choice([32, 48, 64, 96]))
+        P = int(rng.choice([11, 16]))
+        per = 2 if rng.random() < se2p_frac else 3
+        x = rng.random(3 * N)
+        q = np.ones(N)
+        q[N // 2:] = -1.0
+        if N % 2:
+            q[-1] = 0.0
+        rng.shuffle(q)
+        out.append(dict(N=N, M=M, P=P, xi=0.3125 * M, per=per, x=x, q=q))
+    return out
+
+
+def _device_jobs(systems, prec):
+    import torch
+    import se_fast_gridding_amd as se
+    jobs = []
+    for s in systems:
+        x = torch.tensor(s["x"], device="cuda")
+        q = torch.tensor(s["q"], device="cuda")
+        phi = torch.zeros_like(q)
+        job = se.Batch.job(s["M"], s["P"], 1.0, s["xi"], x, q, phi, precision=prec, periodicity=s["per"],
+                           slab=1.0 if s["per"] == 2 else 0.0)
+        jobs.append((job, x, q, phi))
+    return jobs
+
+
+def run_batched(args):
+    """512 independent systems: FP32 jobs over `slots` streams vs the same systems solved one
+    after another on the FP64 path (plans cached per shape, one stream)."""
+    import numpy as np
+    import torch
+    import se_fast_gridding_amd as se
+    n_sys = args.n or 512
+    systems = mixed_systems(n_sys)
+    upd = sum(2.0 * s["N"] * s["P"] ** 3 for s in systems)
+
+    def timed(prec, slots):
+        jobs = _device_jobs(systems, prec)
+        b = se.Batch(slots=slots)
+        for j in jobs:                       # warm-up: plans for every shape on the slots
+            b.submit(j[0])
+        b.wait_all()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for j in jobs:
+                b.submit(j[0])
+            b.wait_all()
+        dt = (time.perf_counter() - t0) / args.steps
+        phis = [j[3].cpu().numpy() for j in jobs]
+        st = b.stats()
+        b.close()
+        return dt, phis, st
+
+    t32, phi32, st32 = timed(se.PREC_FP32, args.slots)
+    t64, phi64, _ = timed(se.PREC_FP64, 1)
+    errs = np.array([float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)) for a, b in zip(phi32, phi64)])
+    out = {
+        "metric": "batched solves/s", "value": n_sys / t32, "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": t32 * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%d independent SE3P potential systems: N log-uniform in [1e3, 1e5], "
+                               "M in {32,48,64,96}^3, xi = 0.3125 M, P in {11,16}, q = +-1 net zero" % n_sys,
+                   "slots": args.slots, "particles": int(sum(s["N"] for s in systems))},
+        "updates_per_s": upd / t32,
+        "baseline_fp64_sequential": {"solves_per_s": n_sys / t64, "ms": t64 * 1e3},
+        "speedup_vs_fp64_sequential": t64 / t32,
+        "phi_rms_rel_fp32_vs_fp64": {"max": float(errs.max()), "median": float(np.median(errs))},
+        "plan_cache": st32,
+    }
+    print(json.dumps(out))
+
+
+def run_serving(args):
+    """Poisson arrivals of SE3P / SE2P potential solves (50 / 50, sizes as the batched mix)
+    against the executor: a request is admitted when a slot is idle (try_submit), results
+    and inputs stay in HBM; latency = completion seen by the host - arrival."""
+    import collections
+    import numpy as np
+    import torch
+    import se_fast_gridding_amd as se
+    pool = mixed_systems(128, seed=5, se2p_frac=0.5)
+    jobs = _device_jobs(pool, se.PREC_FP32)
+    b = se.Batch(slots=args.slots)
+    biggest = {}                                 # warm-up: every shape's plan on every slot
+    for j in jobs:
+        sh = j[0].shape
+        k = (sh.periodicity, sh.M, sh.P)
+        if k not in biggest or j[0].N > biggest[k].N:
+            biggest[k] = j[0]
+    for job in biggest.values():
+        b.prepare(job)
+    for j in jobs:
+        b.submit(j[0])
+    b.wait_all()
+    rates = [float(r) for r in args.rates.split(",")]
+    rng = np.random.default_rng(5)
+    table = []
+    for lam in rates:
+        K = int(min(max(400, lam * args.serve_seconds), 20000))
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, K))
+        busy = {}                                # pool index -> ticket in flight
+        inflight = {}                            # ticket -> (request, arrival)
+        queue = collections.deque()
+        lat = []
+        i = 0
+        t_start = time.perf_counter()
+        while len(lat) < K:
+            now = time.perf_counter() - t_start
+            for t in list(inflight):
+                if b.poll(t):
+                    r, a = inflight.pop(t)
+                    lat.append(now - a)
+                    busy.pop(r % len(jobs), None)
+            while i < K and arrivals[i] <= now:
+                queue.append(i)
+                i += 1
+            while queue:
+                r = queue[0]
+                if (r % len(jobs)) in busy:      # that system's buffers are still in use
+                    break
+                t = b.try_submit(jobs[r % len(jobs)][0])
+                if t is None:
+                    break
+                inflight[t] = (r, arrivals[r])
+                busy[r % len(jobs)] = t
+                queue.popleft()
+        t_end = time.perf_counter() - t_start
+        lat = np.array(lat) * 1e3
+        table.append({"rate_req_s": lam, "requests": K, "solves_per_s": K / t_end,
+                      "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                      "mean_ms": float(lat.mean())})
+    b.close()
+    best = max(r["solves_per_s"] for r in table)
+    out = {
+        "metric": "served solves/s", "value": best, "unit": "solves/s", "n_gpus": 1, "steps": len(rates),
+        "warmup": 2, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "Poisson arrivals of SE3P/SE2P potential solves (50/50), sizes as the batched mix, "
+                               "128 device-resident systems", "slots": args.slots},
+        "rates": table,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="uniform",
-                    help="uniform: N=1M, M=128, xi=40 (the headline line); clustered: N=4M, M=256, xi=80")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batched", "serving"], default="uniform",
+                    help="uniform: N=1M, M=128, xi=40 (the headline line); clustered: N=4M, M=256, xi=80; "
+                         "batched: 512 independent systems; serving: Poisson request stream")
+    ap.add_argument("--slots", type=int, default=16, help="executor streams (batched / serving)")
+    ap.add_argument("--rates", default="100,200,500,1000,2000,5000", help="serving arrival rates, req/s")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="target duration per serving rate")
     ap.add_argument("--n", type=int, default=0, help="particles (0 = the workload's)")
     ap.add_argument("--m", type=int, default=0, help="grid side (0 = the workload's)")
     ap.add_argument("--p", type=int, default=0, help="Gaussian support (0 = the workload's)")
@@ -276,7 +435,12 @@ def main():
         print("bench.py measures one GPU: every box in this setup has exactly one; "
               "multi-GPU sharding is a design note, not a measured path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
-    run(args)
+    if args.workload == "batched":
+        run_batched(args)
+    elif args.workload == "serving":
+        run_serving(args)
+    else:
+        run(args)
 
 
 if __name__ == "__main__":

@@ -85,6 +85,14 @@ int se_plan_grid_shape(const se_plan* plan, int dims[3], double* z0);
 /* Full k-space solve.  phi and/or force may be NULL (not computed). */
 int se_solve(se_plan* plan, int N, const double* d_x, const double* d_q, double* d_phi, double* d_force);
 
+/* se_solve replayed from a HIP graph: the first call with a given (N, d_x, d_q, d_phi,
+ * d_force) captures the solve on the plan's stream, later calls with the same arguments
+ * launch the captured graph (one launch instead of one per kernel).  The plan keeps the
+ * 32 most recently used graphs.  The arrays' contents may change between calls; the
+ * arguments may not (a different argument set captures a new graph).  Falls back to
+ * se_solve while timing is enabled. */
+int se_solve_graph(se_plan* plan, int N, const double* d_x, const double* d_q, double* d_phi, double* d_force);
+
 /* Stages of se_solve, for timing and for callers that bring their own grid. */
 int se_spread(se_plan* plan, int N, const double* d_x, const double* d_q);
 int se_kspace(se_plan* plan);
@@ -114,6 +122,57 @@ int se_plan_info(se_plan* plan, int out[8]);
 int se_energy(se_plan* plan, int N, const double* d_phi, const double* d_q, double* d_energy);
 
 const char* se_gpu_last_error(void);
+
+/*
+ * Batched and served solves (no reference counterpart: the reference solves one system
+ * per call).  An executor owns `n_slots` HIP streams and a per-slot cache of plans
+ * keyed by geometry; jobs are independent systems in device memory.  se_batch_submit
+ * queues a job on the slots in turn; se_batch_try_submit takes an idle slot (one that
+ * already holds a plan for the job first) or returns SE_BUSY (the admission step of a
+ * serving loop).  se_batch_submit replays the job through se_solve_graph (captured on the
+ * first submission of the same arguments); se_batch_try_submit solves eagerly.
+ * Plans of all slots together
+ * stay within `hbm_budget` bytes (0 = 64 GiB); least recently used plans of idle slots
+ * are freed to make room, and a job whose plan cannot fit fails with SE_ERR_CAPACITY.
+ * Jobs on one slot run in order, jobs on different slots overlap.  A job's result does
+ * not depend on its slot or its neighbours beyond rounding: the only run-to-run freedom
+ * is the order of particles inside one sort bucket (atomic ranks) and of LDS atomics
+ * inside one work item, so repeated solves agree to about 1e-16 (FP64 plans) or to FP32
+ * rounding (FP32 plans).
+ * The executor is not thread-safe; the caller serialises calls.
+ */
+#define SE_BUSY 1
+
+typedef struct {
+    int periodicity;   /* 3 or 2 */
+    int M, P;
+    double box, xi, m; /* m = 0: 0.95 sqrt(pi P) */
+    double slab;       /* SE2P slab height */
+    int precision;     /* SE_PREC_FP64 / SE_PREC_FP32 */
+} se_job_shape;
+
+typedef struct {
+    se_job_shape shape;
+    int N;
+    const double* d_x; /* SoA [3N], device */
+    const double* d_q; /* [N], device */
+    double* d_phi;     /* [N] or NULL */
+    double* d_force;   /* [3N] or NULL */
+} se_job;
+
+typedef struct se_batch se_batch;
+int se_batch_create(se_batch** batch, int n_slots, size_t hbm_budget);
+int se_batch_destroy(se_batch* batch);
+int se_batch_submit(se_batch* batch, const se_job* job, int* ticket);
+int se_batch_try_submit(se_batch* batch, const se_job* job, int* ticket);
+/* Build (or grow) the plan for `shape` with capacity n_max on every slot ahead of time. */
+int se_batch_prepare(se_batch* batch, const se_job_shape* shape, int n_max);
+/* 1 when the ticket's job is complete, 0 while it runs, < 0 on error. */
+int se_batch_poll(se_batch* batch, int ticket);
+int se_batch_wait(se_batch* batch, int ticket);
+int se_batch_wait_all(se_batch* batch);
+/* HBM held by cached plans and their count. */
+int se_batch_stats(se_batch* batch, size_t* plan_bytes, int* n_plans);
 
 #ifdef __cplusplus
 }

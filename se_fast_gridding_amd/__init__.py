@@ -20,6 +20,7 @@ ROOT = os.path.dirname(HERE)
 INCLUDE_DIR = os.path.join(ROOT, "include")
 
 SE_OK, SE_ERR_ARG, SE_ERR_HIP, SE_ERR_FFT, SE_ERR_CAPACITY, SE_ERR_RANGE = 0, -1, -2, -3, -4, -5
+SE_BUSY = 1
 PREC_FP64, PREC_FP32 = 0, 1
 Q_POTENTIAL, Q_FORCE = 1, 2
 
@@ -64,6 +65,18 @@ class se_plan_desc(C.Structure):
                 ("slab", C.c_double), ("Mz", C.c_int)]
 
 PLAN_GENERIC_KERNELS = 1
+
+
+class se_job_shape(C.Structure):
+    """se_gpu.h se_job_shape."""
+    _fields_ = [("periodicity", C.c_int), ("M", C.c_int), ("P", C.c_int), ("box", C.c_double),
+                ("xi", C.c_double), ("m", C.c_double), ("slab", C.c_double), ("precision", C.c_int)]
+
+
+class se_job(C.Structure):
+    """se_gpu.h se_job."""
+    _fields_ = [("shape", se_job_shape), ("N", C.c_int), ("d_x", C.c_void_p), ("d_q", C.c_void_p),
+                ("d_phi", C.c_void_p), ("d_force", C.c_void_p)]
 
 
 _D = C.POINTER(C.c_double)
@@ -129,6 +142,7 @@ _SIG = {
     "se_plan_bytes": (C.c_size_t, [_VP]),
     "se_plan_bytes_for": (C.c_size_t, [C.POINTER(se_plan_desc)]),
     "se_solve": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP]),
+    "se_solve_graph": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP]),
     "se_spread": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "se_kspace": (C.c_int, [_VP]),
     "se_interp": (C.c_int, [_VP, _VP, _VP]),
@@ -139,6 +153,15 @@ _SIG = {
     "se_plan_info": (C.c_int, [_VP, _I]),
     "se_energy": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
     "se_gpu_last_error": (C.c_char_p, []),
+    "se_batch_create": (C.c_int, [C.POINTER(_VP), C.c_int, C.c_size_t]),
+    "se_batch_destroy": (C.c_int, [_VP]),
+    "se_batch_submit": (C.c_int, [_VP, C.POINTER(se_job), _I]),
+    "se_batch_try_submit": (C.c_int, [_VP, C.POINTER(se_job), _I]),
+    "se_batch_prepare": (C.c_int, [_VP, C.POINTER(se_job_shape), C.c_int]),
+    "se_batch_poll": (C.c_int, [_VP, C.c_int]),
+    "se_batch_wait": (C.c_int, [_VP, C.c_int]),
+    "se_batch_wait_all": (C.c_int, [_VP]),
+    "se_batch_stats": (C.c_int, [_VP, C.POINTER(C.c_size_t), _I]),
 }
 
 _SPREAD_SPLIT = ["SE_FGG_grid_split_SSE_dispatch", "SE_FGG_grid_split", "SE_FGG_grid_split_SSE",
@@ -238,6 +261,11 @@ class Plan:
         N = q.numel()
         check(capi().se_solve(self.h, N, _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_solve")
 
+    def solve_graph(self, x, q, phi=None, force=None):
+        """se_solve through a captured HIP graph (captured on the first call with these tensors)."""
+        N = q.numel()
+        check(capi().se_solve_graph(self.h, N, _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_solve_graph")
+
     def spread(self, x, q):
         check(capi().se_spread(self.h, q.numel(), _ptr(x), _ptr(q)), "se_spread")
 
@@ -304,3 +332,64 @@ def _hip_copy(dst, src, nbytes):
     rc = _hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, 3)   # hipMemcpyDeviceToDevice
     if rc != 0:
         raise RuntimeError("hipMemcpy failed (%d)" % rc)
+
+
+class Batch:
+    """Executor for many independent systems (include/se_gpu.h se_batch_*): `slots` HIP
+    streams with per-slot plan caches under an HBM budget.  Jobs take torch CUDA tensors
+    in the Plan layout; they must stay alive until their ticket completes."""
+
+    def __init__(self, slots=8, hbm_budget=0):
+        h = C.c_void_p()
+        check(capi().se_batch_create(C.byref(h), int(slots), int(hbm_budget)), "se_batch_create")
+        self.h = h
+        self.slots = slots
+
+    def close(self):
+        if getattr(self, "h", None):
+            capi().se_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @staticmethod
+    def job(M, P, box, xi, x, q, phi=None, force=None, precision=PREC_FP32, m=0.0, periodicity=3, slab=0.0):
+        sh = se_job_shape(periodicity, M, P, float(box), float(xi), float(m), float(slab), precision)
+        return se_job(sh, q.numel(), x.data_ptr(), q.data_ptr(), phi.data_ptr() if phi is not None else None,
+                      force.data_ptr() if force is not None else None)
+
+    def submit(self, job):
+        t = C.c_int()
+        check(capi().se_batch_submit(self.h, C.byref(job), C.byref(t)), "se_batch_submit")
+        return t.value
+
+    def try_submit(self, job):
+        """Ticket, or None when no slot is idle."""
+        t = C.c_int()
+        rc = capi().se_batch_try_submit(self.h, C.byref(job), C.byref(t))
+        if rc == SE_BUSY:
+            return None
+        check(rc, "se_batch_try_submit")
+        return t.value
+
+    def prepare(self, job):
+        """Build the plan for this job's shape (capacity at least its N) on every slot."""
+        check(capi().se_batch_prepare(self.h, C.byref(job.shape), job.N), "se_batch_prepare")
+
+    def poll(self, ticket):
+        rc = capi().se_batch_poll(self.h, int(ticket))
+        if rc < 0:
+            check(rc, "se_batch_poll")
+        return rc == 1
+
+    def wait(self, ticket):
+        check(capi().se_batch_wait(self.h, int(ticket)), "se_batch_wait")
+
+    def wait_all(self):
+        check(capi().se_batch_wait_all(self.h), "se_batch_wait_all")
+
+    def stats(self):
+        b = C.c_size_t()
+        n = C.c_int()
+        check(capi().se_batch_stats(self.h, C.byref(b), C.byref(n)), "se_batch_stats")
+        return {"plan_bytes": int(b.value), "plans": int(n.value)}

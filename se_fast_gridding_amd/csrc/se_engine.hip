@@ -126,6 +126,15 @@ struct se_plan {
     bool timing = false;
     hipEvent_t ev[8] = {};
     std::vector<void*> allocs;
+    // captured solves, keyed by their arguments (see se_solve_graph)
+    struct GraphEntry {
+        int N;
+        const void *x, *q, *phi, *force;
+        hipGraphExec_t exec;
+        unsigned long last_use;
+    };
+    std::vector<GraphEntry> graphs;
+    unsigned long graph_clock = 0;
 
     void* alloc(size_t b)
     {
@@ -138,6 +147,7 @@ struct se_plan {
     }
     ~se_plan()
     {
+        for (auto& gph : graphs) (void)hipGraphExecDestroy(gph.exec);
         if (fwd) rocfft_plan_destroy(fwd);
         if (bwd) rocfft_plan_destroy(bwd);
         if (info) rocfft_execution_info_destroy(info);
@@ -537,6 +547,52 @@ int se_solve(se_plan* pl, int N, const double* x, const double* q, double* phi, 
     rc = se_kspace(pl);
     if (rc) return rc;
     return se_interp(pl, phi, force);
+}
+
+int se_solve_graph(se_plan* pl, int N, const double* x, const double* q, double* phi, double* force)
+{
+    if (!pl) return SE_ERR_ARG;
+    if (pl->timing) return se_solve(pl, N, x, q, phi, force);   // events do not go into graphs
+    for (auto& e : pl->graphs)
+        if (e.N == N && e.x == x && e.q == q && e.phi == phi && e.force == force) {
+            e.last_use = ++pl->graph_clock;
+            return guarded([&] { SE_HIP(hipGraphLaunch(e.exec, pl->stream)); });
+        }
+    // validate eagerly (argument errors must not happen inside a capture)
+    if (N < 0 || N > pl->n_cap) {
+        se::set_error("N exceeds the plan's particle capacity");
+        return SE_ERR_CAPACITY;
+    }
+    if (N > 0 && (!x || !q)) return SE_ERR_ARG;
+    int rc = SE_OK;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    rc = guarded([&] { SE_HIP(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal)); });
+    if (rc) return rc;
+    const int src = se_solve(pl, N, x, q, phi, force);
+    const hipError_t ec = hipStreamEndCapture(pl->stream, &graph);
+    if (src != SE_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return src;
+    }
+    rc = guarded([&] {
+        SE_HIP(ec);
+        SE_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        graph = nullptr;
+        const size_t max_graphs = 32;
+        if (pl->graphs.size() >= max_graphs) {
+            auto lru = std::min_element(pl->graphs.begin(), pl->graphs.end(),
+                                        [](const auto& a, const auto& b) { return a.last_use < b.last_use; });
+            SE_HIP(hipStreamSynchronize(pl->stream));
+            (void)hipGraphExecDestroy(lru->exec);
+            pl->graphs.erase(lru);
+        }
+        pl->graphs.push_back({N, x, q, phi, force, exec, ++pl->graph_clock});
+        SE_HIP(hipGraphLaunch(exec, pl->stream));
+    });
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
 }
 
 int se_plan_grid_shape(const se_plan* pl, int dims[3], double* z0)
