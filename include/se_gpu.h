@@ -93,6 +93,30 @@ int se_solve(se_plan* plan, int N, const double* d_x, const double* d_q, double*
  * se_solve while timing is enabled. */
 int se_solve_graph(se_plan* plan, int N, const double* d_x, const double* d_q, double* d_phi, double* d_force);
 
+/*
+ * Grouped solves (no reference counterpart: the reference solves one system per call).
+ * A group plan holds up to `max_systems` independent systems of the plan's geometry
+ * (desc->n_max = particle capacity summed over the systems) and solves them together:
+ * each stage is one launch over all systems (sort keys, work items and partials are
+ * indexed by system * tiles + tile, grids and spectra are stacked, the FFTs are batched).
+ * All systems of one call must request the same outputs (phi for all or none, force for
+ * all or none).  Each system's result equals its se_solve result up to the rounding
+ * freedom stated below for se_batch_*.  se_solve / se_spread on a group plan solve one
+ * system.  Descriptors travel through a pinned staging ring, so se_solve_group is not
+ * for stream capture.
+ */
+typedef struct {
+    int N;
+    const double* d_x;   /* SoA [3N], device */
+    const double* d_q;   /* [N], device */
+    double* d_phi;       /* [N] or NULL */
+    double* d_force;     /* SoA [3N] or NULL */
+} se_system;
+
+int se_plan_create_group(se_plan** plan, const se_plan_desc* desc, int max_systems, void* hip_stream);
+size_t se_plan_bytes_for_group(const se_plan_desc* desc, int max_systems);
+int se_solve_group(se_plan* plan, int n_systems, const se_system* systems);
+
 /* Stages of se_solve, for timing and for callers that bring their own grid. */
 int se_spread(se_plan* plan, int N, const double* d_x, const double* d_q);
 int se_kspace(se_plan* plan);
@@ -165,6 +189,10 @@ int se_batch_create(se_batch** batch, int n_slots, size_t hbm_budget);
 int se_batch_destroy(se_batch* batch);
 int se_batch_submit(se_batch* batch, const se_job* job, int* ticket);
 int se_batch_try_submit(se_batch* batch, const se_job* job, int* ticket);
+/* Solve a set of jobs as grouped solves: jobs with the same shape and the same requested
+ * outputs form one group, solved by a cached group plan (se_solve_group) on one slot;
+ * groups overlap on different slots.  One ticket completes when every job is done. */
+int se_batch_run(se_batch* batch, const se_job* jobs, int n_jobs, int* ticket);
 /* Build (or grow) the plan for `shape` with capacity n_max on every slot ahead of time. */
 int se_batch_prepare(se_batch* batch, const se_job_shape* shape, int n_max);
 /* 1 when the ticket's job is complete, 0 while it runs, < 0 on error. */

@@ -79,6 +79,12 @@ class se_job(C.Structure):
                 ("d_phi", C.c_void_p), ("d_force", C.c_void_p)]
 
 
+class se_system(C.Structure):
+    """se_gpu.h se_system."""
+    _fields_ = [("N", C.c_int), ("d_x", C.c_void_p), ("d_q", C.c_void_p), ("d_phi", C.c_void_p),
+                ("d_force", C.c_void_p)]
+
+
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
 _PP = C.POINTER(SE_FGG_params)
@@ -141,6 +147,9 @@ _SIG = {
     "se_plan_set_stream": (C.c_int, [_VP, _VP]),
     "se_plan_bytes": (C.c_size_t, [_VP]),
     "se_plan_bytes_for": (C.c_size_t, [C.POINTER(se_plan_desc)]),
+    "se_plan_create_group": (C.c_int, [C.POINTER(_VP), C.POINTER(se_plan_desc), C.c_int, _VP]),
+    "se_plan_bytes_for_group": (C.c_size_t, [C.POINTER(se_plan_desc), C.c_int]),
+    "se_solve_group": (C.c_int, [_VP, C.c_int, C.POINTER(se_system)]),
     "se_solve": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP]),
     "se_solve_graph": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP]),
     "se_spread": (C.c_int, [_VP, C.c_int, _VP, _VP]),
@@ -156,6 +165,7 @@ _SIG = {
     "se_batch_create": (C.c_int, [C.POINTER(_VP), C.c_int, C.c_size_t]),
     "se_batch_destroy": (C.c_int, [_VP]),
     "se_batch_submit": (C.c_int, [_VP, C.POINTER(se_job), _I]),
+    "se_batch_run": (C.c_int, [_VP, C.POINTER(se_job), C.c_int, _I]),
     "se_batch_try_submit": (C.c_int, [_VP, C.POINTER(se_job), _I]),
     "se_batch_prepare": (C.c_int, [_VP, C.POINTER(se_job_shape), C.c_int]),
     "se_batch_poll": (C.c_int, [_VP, C.c_int]),
@@ -230,18 +240,26 @@ class Plan:
     x is (3N,) float64 SoA [x..., y..., z...], q is (N,) float64, outputs are float64,
     force SoA (3N,) in the reference's (q/2) grad(phi) convention (SE_fgg.c:2291-2307).
     periodicity=2 solves SE2P: periodic in x and y (period box), free in z with particles
-    in 0 <= z < slab; Mz (0 = automatic) is the zero-padded z grid."""
+    in 0 <= z < slab; Mz (0 = automatic) is the zero-padded z grid.  max_systems > 1
+    makes a group plan (n_max then counts the particles of all systems together) for
+    solve_group."""
 
     def __init__(self, M, P, box, xi, n_max, precision=PREC_FP64, m=0.0, tile=0, chunk=0, block=0,
-                 stream=None, generic=False, periodicity=3, slab=0.0, Mz=0):
+                 stream=None, generic=False, periodicity=3, slab=0.0, Mz=0, max_systems=1):
         import torch
         self._torch = torch
         d = se_plan_desc(periodicity, M, P, float(box), float(xi), float(m), precision, int(n_max), tile, chunk,
                          block, PLAN_GENERIC_KERNELS if generic else 0, float(slab), int(Mz))
         self.desc = d
+        self.max_systems = int(max_systems)
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         h = C.c_void_p()
-        check(capi().se_plan_create(C.byref(h), C.byref(d), C.c_void_p(self.stream.cuda_stream)), "se_plan_create")
+        if self.max_systems > 1:
+            check(capi().se_plan_create_group(C.byref(h), C.byref(d), self.max_systems,
+                                              C.c_void_p(self.stream.cuda_stream)), "se_plan_create_group")
+        else:
+            check(capi().se_plan_create(C.byref(h), C.byref(d), C.c_void_p(self.stream.cuda_stream)),
+                  "se_plan_create")
         self.h = h
         self.M, self.P, self.n_max, self.precision = M, P, n_max, precision
         dims = (C.c_int * 3)()
@@ -260,6 +278,15 @@ class Plan:
     def solve(self, x, q, phi=None, force=None):
         N = q.numel()
         check(capi().se_solve(self.h, N, _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_solve")
+
+    def solve_group(self, systems):
+        """Grouped solve of [(x, q, phi, force), ...] (phi / force may be None, the same
+        for every system) in one pass (se_solve_group)."""
+        arr = (se_system * len(systems))()
+        for i, (x, q, phi, force) in enumerate(systems):
+            arr[i] = se_system(q.numel(), x.data_ptr(), q.data_ptr(), phi.data_ptr() if phi is not None else None,
+                               force.data_ptr() if force is not None else None)
+        check(capi().se_solve_group(self.h, len(systems), arr), "se_solve_group")
 
     def solve_graph(self, x, q, phi=None, force=None):
         """se_solve through a captured HIP graph (captured on the first call with these tensors)."""
@@ -361,6 +388,13 @@ class Batch:
     def submit(self, job):
         t = C.c_int()
         check(capi().se_batch_submit(self.h, C.byref(job), C.byref(t)), "se_batch_submit")
+        return t.value
+
+    def run(self, jobs):
+        """Grouped solve of a list of jobs (se_batch_run); returns one ticket."""
+        arr = (se_job * len(jobs))(*jobs)
+        t = C.c_int()
+        check(capi().se_batch_run(self.h, arr, len(jobs), C.byref(t)), "se_batch_run")
         return t.value
 
     def try_submit(self, job):

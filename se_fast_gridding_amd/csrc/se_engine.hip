@@ -55,7 +55,7 @@ struct DevMem {
 // Tiling rule.  Spread sub-grids accumulate in FP64 LDS (S^3 * 8 bytes), gather
 // sub-grids hold the grid precision.  Default tile side: the largest (<= 16; even
 // and <= 12 for the pencil kernels) whose FP64 sub-grid fits in LDS.
-static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int tile, int chunk, int block,
+static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sys, int tile, int chunk, int block,
                             int lds_max, bool allow_pencil)
 {
     Tiling tl{};
@@ -94,7 +94,7 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int tile
     SE_REQUIRE(tl.block % 64 == 0 && tl.block <= 512, SE_ERR_ARG, "block must be a multiple of 64, <= 512");
     tl.sub_elems = tl.S * tl.S * tl.Szp;
     tl.part_elems = tl.S * tl.S * tl.S;
-    tl.max_items = tl.ntiles + n_cap / tl.chunk + 1;
+    tl.max_items = tl.ntiles * n_sys + n_cap / tl.chunk + 1;
     return tl;
 }
 
@@ -112,6 +112,22 @@ struct se_plan {
     hipStream_t stream = nullptr;
     int n_cap = 0;
     int last_N = -1;
+    // grouped solves: up to s_cap systems of this geometry stacked in one pass
+    int s_cap = 1;
+    int last_S = 1;                 // systems of the last spread
+    bool last_group = false;        // the last spread was grouped (the gather reads d_sys)
+    SysDesc* d_sys = nullptr;       // device descriptors of the last grouped spread
+    SysDesc* h_sys[4] = {};         // pinned staging ring for the descriptors
+    hipEvent_t h_ev[4] = {};        // staging buffer i is free once h_ev[i] completed
+    int h_next = 0;
+    struct GroupFft {
+        int S;
+        rocfft_plan fwd, bwd;
+        rocfft_execution_info info;
+        unsigned long last_use;
+    };
+    std::vector<GroupFft> gffts;   // batched transforms, one pair per group size
+    unsigned long gfft_clock = 0;
     // device buffers
     int *key = nullptr, *rank = nullptr, *count = nullptr, *row_start = nullptr, *tile_count = nullptr,
         *nchunk = nullptr, *chunk_start = nullptr, *item_tile = nullptr, *item_beg = nullptr,
@@ -148,6 +164,15 @@ struct se_plan {
     ~se_plan()
     {
         for (auto& gph : graphs) (void)hipGraphExecDestroy(gph.exec);
+        for (auto& f : gffts) {
+            rocfft_plan_destroy(f.fwd);
+            rocfft_plan_destroy(f.bwd);
+            rocfft_execution_info_destroy(f.info);
+        }
+        for (int i = 0; i < 4; ++i) {
+            if (h_ev[i]) (void)hipEventDestroy(h_ev[i]);
+            if (h_sys[i]) (void)hipHostFree(h_sys[i]);
+        }
         if (fwd) rocfft_plan_destroy(fwd);
         if (bwd) rocfft_plan_destroy(bwd);
         if (info) rocfft_execution_info_destroy(info);
@@ -291,27 +316,28 @@ void build_merge_segments(se_plan* pl)
     upload(lz, pl->segz, pl->nsegz);
 }
 
-size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max)
+size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
 {
     Geom g;
     double eta, m;
     double R2p = 0;
     derive(d, g, eta, m, R2p);
     const int es = d->precision == SE_PREC_FP32 ? 4 : 8;
-    Tiling tl = choose_tiling(es, d->P, d->M, g.Mz, d->n_max, d->tile, d->chunk, d->block, lds_max,
+    Tiling tl = choose_tiling(es, d->P, d->M, g.Mz, d->n_max, n_sys, d->tile, d->chunk, d->block, lds_max,
                               !(d->flags & SE_PLAN_GENERIC_KERNELS));
-    const size_t M = d->M, Mz = g.Mz, N = d->n_max;
+    const size_t M = d->M, Mz = g.Mz, N = d->n_max, ns = n_sys;
     size_t b = 0;
-    b += 2 * N * 4;                                   // key, rank
-    b += 2 * (size_t)(tl.nbins + 1) * 4;              // count, row_start
-    b += 4 * (size_t)(tl.ntiles + 2) * 4;             // tile_count, nchunk, chunk_start
-    b += 3 * (size_t)tl.max_items * 4;                // items
-    b += N * (es == 4 ? 32 : 48);                     // records
-    b += (size_t)tl.max_items * tl.part_elems * es;   // partials
-    b += M * M * Mz * es;                             // grid
-    b += M * M * (Mz / 2 + 1) * 2 * es;               // half spectrum
-    b += M * M * (Mz / 2 + 1) * 2 * es;               // FFT work (upper estimate)
-    if (g.per == 2) b += M * M * (Mz / 2 + 1) * 8;    // SE2P Green's function table
+    b += 2 * N * 4;                                       // key, rank
+    b += 2 * (size_t)(ns * tl.nbins + 1) * 4;             // count, row_start
+    b += 4 * (size_t)(ns * tl.ntiles + 2) * 4;            // tile_count, nchunk, chunk_start
+    b += 3 * (size_t)tl.max_items * 4;                    // items
+    b += N * (es == 4 ? 32 : 48);                         // records
+    b += (size_t)tl.max_items * tl.part_elems * es;       // partials
+    b += ns * M * M * Mz * es;                            // grids
+    b += ns * M * M * (Mz / 2 + 1) * 2 * es;              // half spectra
+    b += ns * M * M * (Mz / 2 + 1) * 2 * es;              // FFT work (upper estimate)
+    if (g.per == 2) b += M * M * (Mz / 2 + 1) * 8;        // SE2P Green's function table
+    if (ns > 1) b += ns * sizeof(SysDesc);                // descriptors
     return b;
 }
 
@@ -324,81 +350,188 @@ int lds_limit()
     return v;
 }
 
+// Spread stage.  S == 0: one system (x, q, N).  S > 0: a grouped solve of S systems whose
+// descriptors are in pl->d_sys and whose particle kernels take `blocks` blocks.
 template <typename T>
-void run_spread(se_plan* pl, int N, const double* x, const double* q)
+void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int blocks)
 {
     hipStream_t s = pl->stream;
     const Tiling& tl = pl->tl;
+    const bool grp = S > 0;
+    const int ns = grp ? S : 1;
+    const long nb = (long)ns * tl.nbins;
+    const int nt = ns * tl.ntiles;
     pl->rec_ev(0);
-    launch_fill_i32(pl->count, (long)tl.nbins + 1, 0, s);
-    launch_bin_keys<T>(x, N, pl->g, tl, pl->key, pl->rank, pl->count, s);
+    launch_fill_i32(pl->count, nb + 1, 0, s);
+    if (grp)
+        launch_bin_keys_group<T>(pl->d_sys, S, blocks, pl->g, tl, pl->key, pl->rank, pl->count, s);
+    else
+        launch_bin_keys<T>(x, N, pl->g, tl, pl->key, pl->rank, pl->count, s);
     size_t tb = pl->cub_bytes;
-    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->count, pl->row_start, tl.nbins + 1, s));
-    launch_tile_chunks(pl->row_start, tl.ntiles, tl.K, tl.chunk, pl->tile_count, pl->nchunk, s);
+    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->count, pl->row_start, (int)(nb + 1), s));
+    launch_tile_chunks(pl->row_start, nt, tl.K, tl.chunk, pl->tile_count, pl->nchunk, s);
     tb = pl->cub_bytes;
-    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->nchunk, pl->chunk_start, tl.ntiles + 1, s));
-    launch_items(pl->tile_count, pl->row_start, pl->chunk_start, tl.ntiles, tl.K, tl.chunk, pl->item_tile,
-                 pl->item_beg, pl->item_end, pl->n_items, s);
-    launch_scatter<T>(x, q, N, pl->g, tl, pl->key, pl->rank, pl->row_start, (Rec<T>*)pl->rec, s);
+    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->nchunk, pl->chunk_start, nt + 1, s));
+    launch_items(pl->tile_count, pl->row_start, pl->chunk_start, nt, tl.K, tl.chunk, pl->item_tile, pl->item_beg,
+                 pl->item_end, pl->n_items, s);
+    if (grp)
+        launch_scatter_group<T>(pl->d_sys, S, blocks, pl->g, tl, pl->key, pl->rank, pl->row_start, (Rec<T>*)pl->rec,
+                                s);
+    else
+        launch_scatter<T>(x, q, N, pl->g, tl, pl->key, pl->rank, pl->row_start, (Rec<T>*)pl->rec, s);
     pl->rec_ev(1);
+    // work items: at most one partial item per non-empty tile plus one per full chunk
+    Tiling tlc = tl;
+    tlc.max_items = std::min(tl.max_items, nt + N / tl.chunk + 1);
     bool done = false;
     if constexpr (sizeof(T) == 4) {
         if (tl.pencil)
             done = launch_spread_pencil((const Rec<float>*)pl->rec, pl->item_beg, pl->item_end, pl->n_items, pl->g,
-                                        tl, (float*)pl->partials, s);
+                                        tlc, (float*)pl->partials, s);
     }
     if (!done)
         launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
-                               pl->g, tl, (T*)pl->partials, s);
+                               pl->g, tlc, (T*)pl->partials, s);
     pl->rec_ev(2);
     launch_merge<T>((const T*)pl->partials, pl->chunk_start, pl->seg, pl->nseg, pl->segz, pl->nsegz, tl.nseg, pl->g,
-                    tl, (T*)pl->grid, s);
+                    tl, (T*)pl->grid, ns, s);
     pl->rec_ev(3);
+    pl->last_N = N;
+    pl->last_S = ns;
+    pl->last_group = grp;
     SE_HIP(hipGetLastError());
 }
 
+// Gather stage.  After a grouped spread, phi / force only select the output mode and
+// every system's outputs come from the descriptors.
 template <typename T>
 void run_interp(se_plan* pl, double* phi, double* force)
 {
     pl->rec_ev(4);
+    const SysDesc* sys = pl->last_group ? pl->d_sys : nullptr;
+    Tiling tlc = pl->tl;
+    tlc.max_items = std::min(pl->tl.max_items, pl->last_S * pl->tl.ntiles + pl->last_N / pl->tl.chunk + 1);
     bool done = false;
     if constexpr (sizeof(T) == 4) {
         if (pl->tl.pencil)
             done = launch_interp_pencil((const Rec<float>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end,
-                                        pl->n_items, (const float*)pl->grid, pl->g, pl->tl, pl->last_N, phi, force,
+                                        pl->n_items, (const float*)pl->grid, pl->g, tlc, pl->last_N, phi, force, sys,
                                         pl->stream);
     }
     if (!done)
         launch_interp_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
-                               (const T*)pl->grid, pl->g, pl->tl, pl->last_N, phi, force, pl->stream);
+                               (const T*)pl->grid, pl->g, tlc, pl->last_N, phi, force, sys, pl->stream);
     pl->rec_ev(5);
     SE_HIP(hipGetLastError());
 }
 
+void make_fft_pair(se_plan* pl, int S, rocfft_plan& fwd, rocfft_plan& bwd, size_t& work);
+se_plan::GroupFft& group_fft(se_plan* pl, int S);
+
+// k-space stage over the grids of the last spread (one, or S stacked ones with batched
+// transforms).
 void run_kspace(se_plan* pl)
 {
     pl->rec_ev(6);
+    const int ns = pl->last_S;
+    rocfft_plan fwd = pl->fwd, bwd = pl->bwd;
+    rocfft_execution_info info = pl->info;
+    if (ns > 1) {
+        se_plan::GroupFft& f = group_fft(pl, ns);
+        fwd = f.fwd;
+        bwd = f.bwd;
+        info = f.info;
+    }
     void* in[1] = {pl->grid};
     void* out[1] = {pl->spec};
-    SE_FFT(rocfft_execute(pl->fwd, in, out, pl->info));
+    SE_FFT(rocfft_execute(fwd, in, out, info));
     const int M = pl->g.M, Mz = pl->g.Mz;
     if (pl->g.per == 3) {
         if (pl->esize == 4)
-            launch_green<float>(pl->spec, M, Mz, pl->gtab, pl->stream);
+            launch_green<float>(pl->spec, M, Mz, pl->gtab, ns, pl->stream);
         else
-            launch_green<double>(pl->spec, M, Mz, pl->gtab, pl->stream);
+            launch_green<double>(pl->spec, M, Mz, pl->gtab, ns, pl->stream);
     } else {
         const long n = (long)M * M * (Mz / 2 + 1);
         if (pl->esize == 4)
-            launch_green_full<float>(pl->spec, n, pl->gtab, pl->stream);
+            launch_green_full<float>(pl->spec, n, pl->gtab, ns, pl->stream);
         else
-            launch_green_full<double>(pl->spec, n, pl->gtab, pl->stream);
+            launch_green_full<double>(pl->spec, n, pl->gtab, ns, pl->stream);
     }
     void* in2[1] = {pl->spec};
     void* out2[1] = {pl->grid};
-    SE_FFT(rocfft_execute(pl->bwd, in2, out2, pl->info));
+    SE_FFT(rocfft_execute(bwd, in2, out2, info));
     pl->rec_ev(7);
     SE_HIP(hipGetLastError());
+}
+
+// R2C / C2R plans for S stacked grids, lengths fastest-first (z, y, x).
+void make_fft_pair(se_plan* pl, int S, rocfft_plan& fwd, rocfft_plan& bwd, size_t& work)
+{
+    const size_t M = pl->g.M, Mz = pl->g.Mz;
+    size_t lens[3] = {Mz, M, M};
+    const rocfft_precision prec = pl->esize == 4 ? rocfft_precision_single : rocfft_precision_double;
+    const size_t rdist = M * M * Mz, cdist = M * M * (Mz / 2 + 1);
+    size_t rstr[3] = {1, Mz, M * Mz}, cstr[3] = {1, Mz / 2 + 1, M * (Mz / 2 + 1)};
+    rocfft_plan_description df = nullptr, db = nullptr;
+    SE_FFT(rocfft_plan_description_create(&df));
+    SE_FFT(rocfft_plan_description_create(&db));
+    struct Guard {
+        rocfft_plan_description a, b;
+        ~Guard()
+        {
+            rocfft_plan_description_destroy(a);
+            rocfft_plan_description_destroy(b);
+        }
+    } gd{df, db};
+    SE_FFT(rocfft_plan_description_set_data_layout(df, rocfft_array_type_real, rocfft_array_type_hermitian_interleaved,
+                                                   nullptr, nullptr, 3, rstr, rdist, 3, cstr, cdist));
+    SE_FFT(rocfft_plan_description_set_data_layout(db, rocfft_array_type_hermitian_interleaved, rocfft_array_type_real,
+                                                   nullptr, nullptr, 3, cstr, cdist, 3, rstr, rdist));
+    SE_FFT(rocfft_plan_create(&fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, prec, 3, lens,
+                              (size_t)S, df));
+    SE_FFT(rocfft_plan_create(&bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, prec, 3, lens,
+                              (size_t)S, db));
+    size_t w1 = 0, w2 = 0;
+    SE_FFT(rocfft_plan_get_work_buffer_size(fwd, &w1));
+    SE_FFT(rocfft_plan_get_work_buffer_size(bwd, &w2));
+    work = std::max(w1, w2);
+}
+
+// Batched transforms for S grids, cached per size (the 8 most recently used sizes).
+se_plan::GroupFft& group_fft(se_plan* pl, int S)
+{
+    for (auto& f : pl->gffts)
+        if (f.S == S) {
+            f.last_use = ++pl->gfft_clock;
+            return f;
+        }
+    if (pl->gffts.size() >= 8) {
+        auto lru = std::min_element(pl->gffts.begin(), pl->gffts.end(),
+                                    [](const auto& a, const auto& b) { return a.last_use < b.last_use; });
+        SE_HIP(hipStreamSynchronize(pl->stream));
+        rocfft_plan_destroy(lru->fwd);
+        rocfft_plan_destroy(lru->bwd);
+        rocfft_execution_info_destroy(lru->info);
+        pl->gffts.erase(lru);
+    }
+    se_plan::GroupFft f{S, nullptr, nullptr, nullptr, ++pl->gfft_clock};
+    size_t work = 0;
+    make_fft_pair(pl, S, f.fwd, f.bwd, work);
+    SE_FFT(rocfft_execution_info_create(&f.info));
+    if (work > pl->fft_work_bytes) {
+        // the plan's FFT work area was sized for the largest group at creation; a size
+        // that needs more gets a larger area (after the stream's work drains)
+        SE_HIP(hipStreamSynchronize(pl->stream));
+        pl->fft_work = pl->alloc(work);
+        pl->fft_work_bytes = work;
+        SE_FFT(rocfft_execution_info_set_work_buffer(pl->info, pl->fft_work, pl->fft_work_bytes));
+        for (auto& o : pl->gffts) SE_FFT(rocfft_execution_info_set_work_buffer(o.info, pl->fft_work, work));
+    }
+    if (pl->fft_work_bytes) SE_FFT(rocfft_execution_info_set_work_buffer(f.info, pl->fft_work, pl->fft_work_bytes));
+    SE_FFT(rocfft_execution_info_set_stream(f.info, pl->stream));
+    pl->gffts.push_back(f);
+    return pl->gffts.back();
 }
 
 }  // namespace
@@ -410,17 +543,31 @@ const char* se_gpu_last_error(void) { return se::last_error(); }
 size_t se_plan_bytes_for(const se_plan_desc* desc)
 {
     size_t b = 0;
-    int rc = guarded([&] { b = plan_bytes_estimate(desc, lds_limit()); });
+    int rc = guarded([&] { b = plan_bytes_estimate(desc, lds_limit(), 1); });
     return rc == SE_OK ? b : 0;
 }
 
-int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
+size_t se_plan_bytes_for_group(const se_plan_desc* desc, int max_systems)
+{
+    size_t b = 0;
+    int rc = guarded([&] {
+        SE_REQUIRE(max_systems >= 1, SE_ERR_ARG, "max_systems must be >= 1");
+        b = plan_bytes_estimate(desc, lds_limit(), max_systems);
+    });
+    return rc == SE_OK ? b : 0;
+}
+
+}  // extern "C"
+
+static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n_sys)
 {
     if (!out || !d) return SE_ERR_ARG;
     *out = nullptr;
     se_plan* pl = new se_plan();
     int rc = guarded([&] {
         SE_REQUIRE(d->n_max >= 0, SE_ERR_ARG, "n_max must be >= 0");
+        SE_REQUIRE(n_sys >= 1 && n_sys <= 65535, SE_ERR_ARG, "max_systems must be in [1, 65535]");
+        pl->s_cap = n_sys;
         pl->desc = *d;
         pl->esize = d->precision == SE_PREC_FP32 ? 4 : 8;
         derive(d, pl->g, pl->eta, pl->m, pl->R2p);
@@ -429,28 +576,37 @@ int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
         pl->stream = (hipStream_t)stream;
         pl->n_cap = std::max(d->n_max, 1);
         const int lds_max = lds_limit();
-        pl->tl = choose_tiling(pl->esize, d->P, d->M, pl->g.Mz, pl->n_cap, d->tile, d->chunk, d->block, lds_max,
-                               !(d->flags & SE_PLAN_GENERIC_KERNELS));
+        pl->tl = choose_tiling(pl->esize, d->P, d->M, pl->g.Mz, pl->n_cap, n_sys, d->tile, d->chunk, d->block,
+                               lds_max, !(d->flags & SE_PLAN_GENERIC_KERNELS));
         const Tiling& tl = pl->tl;
-        const size_t M = d->M, Mz = pl->g.Mz, N = pl->n_cap;
+        const size_t M = d->M, Mz = pl->g.Mz, N = pl->n_cap, ns = n_sys;
+        SE_REQUIRE(ns * tl.nbins < (1ul << 31) - 1 && ns * tl.ntiles * (size_t)tl.K < (1ul << 31) - 1, SE_ERR_ARG,
+                   "too many systems for the sort keys of this geometry");
         pl->key = (int*)pl->alloc(N * 4);
         pl->rank = (int*)pl->alloc(N * 4);
-        pl->count = (int*)pl->alloc(((size_t)tl.nbins + 1) * 4);
-        pl->row_start = (int*)pl->alloc(((size_t)tl.nbins + 1) * 4);
-        pl->tile_count = (int*)pl->alloc((tl.ntiles + 1) * 4);
-        pl->nchunk = (int*)pl->alloc((tl.ntiles + 1) * 4);
-        pl->chunk_start = (int*)pl->alloc((tl.ntiles + 2) * 4);
+        pl->count = (int*)pl->alloc((ns * tl.nbins + 1) * 4);
+        pl->row_start = (int*)pl->alloc((ns * tl.nbins + 1) * 4);
+        pl->tile_count = (int*)pl->alloc((ns * tl.ntiles + 1) * 4);
+        pl->nchunk = (int*)pl->alloc((ns * tl.ntiles + 1) * 4);
+        pl->chunk_start = (int*)pl->alloc((ns * tl.ntiles + 2) * 4);
         pl->item_tile = (int*)pl->alloc((size_t)tl.max_items * 4);
         pl->item_beg = (int*)pl->alloc((size_t)tl.max_items * 4);
         pl->item_end = (int*)pl->alloc((size_t)tl.max_items * 4);
         pl->n_items = (int*)pl->alloc(16);
         pl->rec = pl->alloc(N * (pl->esize == 4 ? sizeof(Rec<float>) : sizeof(Rec<double>)));
         pl->partials = pl->alloc((size_t)tl.max_items * tl.part_elems * pl->esize);
-        pl->grid = pl->alloc(M * M * Mz * pl->esize);
-        pl->spec = pl->alloc(M * M * (Mz / 2 + 1) * 2 * pl->esize);
+        pl->grid = pl->alloc(ns * M * M * Mz * pl->esize);
+        pl->spec = pl->alloc(ns * M * M * (Mz / 2 + 1) * 2 * pl->esize);
         pl->gtab = (double*)pl->alloc((pl->g.per == 3 ? 2 * M + 2 * Mz + 1 : M * M * (Mz / 2 + 1)) * sizeof(double));
-        SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->count, pl->row_start, tl.nbins + 1,
-                                                pl->stream));
+        SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->count, pl->row_start,
+                                                (int)(ns * tl.nbins + 1), pl->stream));
+        if (ns > 1) {
+            pl->d_sys = (SysDesc*)pl->alloc(ns * sizeof(SysDesc));
+            for (int i = 0; i < 4; ++i) {
+                SE_HIP(hipHostMalloc((void**)&pl->h_sys[i], ns * sizeof(SysDesc), hipHostMallocDefault));
+                SE_HIP(hipEventCreateWithFlags(&pl->h_ev[i], hipEventDisableTiming));
+            }
+        }
         pl->cub_tmp = pl->alloc(pl->cub_bytes);
         build_merge_segments(pl);
         // Green's function: s = -(1-eta)/(4 xi^2)   (spectral_ewald.c:126)
@@ -459,18 +615,20 @@ int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
             launch_green_tables(pl->gtab, d->M, (int)Mz, d->box, d->box, gs, pl->stream);
         else
             launch_green2p_table(pl->gtab, d->M, (int)Mz, d->box, pl->g.h, pl->R2p, gs, pl->stream);
-        // rocFFT R2C / C2R, lengths fastest-first (z, y, x)
+        // rocFFT R2C / C2R for one grid; a group plan also sizes the work area for its
+        // largest batch (the batched plans themselves are made per group size on use)
         rocfft_init_once();
-        size_t lens[3] = {Mz, M, M};
-        rocfft_precision prec = pl->esize == 4 ? rocfft_precision_single : rocfft_precision_double;
-        SE_FFT(rocfft_plan_create(&pl->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, prec, 3,
-                                  lens, 1, nullptr));
-        SE_FFT(rocfft_plan_create(&pl->bwd, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, prec, 3,
-                                  lens, 1, nullptr));
-        size_t w1 = 0, w2 = 0;
-        SE_FFT(rocfft_plan_get_work_buffer_size(pl->fwd, &w1));
-        SE_FFT(rocfft_plan_get_work_buffer_size(pl->bwd, &w2));
-        pl->fft_work_bytes = std::max(w1, w2);
+        size_t work = 0;
+        make_fft_pair(pl, 1, pl->fwd, pl->bwd, work);
+        pl->fft_work_bytes = work;
+        if (ns > 1) {
+            rocfft_plan f = nullptr, b = nullptr;
+            size_t wg = 0;
+            make_fft_pair(pl, (int)ns, f, b, wg);
+            rocfft_plan_destroy(f);
+            rocfft_plan_destroy(b);
+            pl->fft_work_bytes = std::max(work, wg);
+        }
         SE_FFT(rocfft_execution_info_create(&pl->info));
         if (pl->fft_work_bytes) {
             pl->fft_work = pl->alloc(pl->fft_work_bytes);
@@ -488,6 +646,15 @@ int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream)
     return SE_OK;
 }
 
+extern "C" {
+
+int se_plan_create(se_plan** out, const se_plan_desc* d, void* stream) { return create_plan(out, d, stream, 1); }
+
+int se_plan_create_group(se_plan** out, const se_plan_desc* d, int max_systems, void* stream)
+{
+    return create_plan(out, d, stream, max_systems);
+}
+
 int se_plan_destroy(se_plan* pl)
 {
     if (!pl) return SE_OK;
@@ -502,6 +669,7 @@ int se_plan_set_stream(se_plan* pl, void* stream)
     return guarded([&] {
         pl->stream = (hipStream_t)stream;
         SE_FFT(rocfft_execution_info_set_stream(pl->info, pl->stream));
+        for (auto& f : pl->gffts) SE_FFT(rocfft_execution_info_set_stream(f.info, pl->stream));
     });
 }
 
@@ -513,11 +681,10 @@ int se_spread(se_plan* pl, int N, const double* x, const double* q)
     return guarded([&] {
         SE_REQUIRE(N >= 0 && N <= pl->n_cap, SE_ERR_CAPACITY, "N exceeds the plan's particle capacity");
         SE_REQUIRE(N == 0 || (x && q), SE_ERR_ARG, "null particle arrays");
-        pl->last_N = N;
         if (pl->esize == 4)
-            run_spread<float>(pl, N, x, q);
+            run_spread<float>(pl, N, x, q, 0, 0);
         else
-            run_spread<double>(pl, N, x, q);
+            run_spread<double>(pl, N, x, q, 0, 0);
     });
 }
 
@@ -547,6 +714,72 @@ int se_solve(se_plan* pl, int N, const double* x, const double* q, double* phi, 
     rc = se_kspace(pl);
     if (rc) return rc;
     return se_interp(pl, phi, force);
+}
+
+int se_solve_group(se_plan* pl, int S, const se_system* sys)
+{
+    if (!pl) return SE_ERR_ARG;
+    return guarded([&] {
+        SE_REQUIRE(S >= 1 && S <= pl->s_cap, SE_ERR_CAPACITY, "number of systems exceeds the group plan's capacity");
+        SE_REQUIRE(sys, SE_ERR_ARG, "null system list");
+        // the output mode comes from the systems that have particles (an empty system has
+        // no outputs, and its arrays may be null)
+        int lead = 0;
+        while (lead < S - 1 && sys[lead].N == 0) ++lead;
+        const bool want_phi = sys[lead].d_phi != nullptr, want_force = sys[lead].d_force != nullptr;
+        long total = 0, blocks = 0;
+        for (int i = 0; i < S; ++i) {
+            SE_REQUIRE(sys[i].N >= 0, SE_ERR_ARG, "negative system size");
+            SE_REQUIRE(sys[i].N == 0 || (sys[i].d_x && sys[i].d_q), SE_ERR_ARG, "null particle arrays");
+            SE_REQUIRE(sys[i].N == 0 || ((sys[i].d_phi != nullptr) == want_phi &&
+                                         (sys[i].d_force != nullptr) == want_force),
+                       SE_ERR_ARG, "all systems of a group must request the same outputs");
+            total += sys[i].N;
+            blocks += (sys[i].N + 255) / 256;
+        }
+        SE_REQUIRE(total <= pl->n_cap, SE_ERR_CAPACITY, "total particles exceed the group plan's capacity");
+        double* phi_mode = sys[lead].d_phi;
+        double* force_mode = sys[lead].d_force;
+        if (S == 1) {   // one system: the plain path, no descriptors
+            if (pl->esize == 4)
+                run_spread<float>(pl, sys[0].N, sys[0].d_x, sys[0].d_q, 0, 0);
+            else
+                run_spread<double>(pl, sys[0].N, sys[0].d_x, sys[0].d_q, 0, 0);
+        } else {
+            // descriptors through a pinned staging ring: buffer i is rewritten only after
+            // the copy that last read it has run
+            const int hb = pl->h_next;
+            pl->h_next = (hb + 1) % 4;
+            SE_HIP(hipEventSynchronize(pl->h_ev[hb]));
+            SysDesc* h = pl->h_sys[hb];
+            long off = 0, blk = 0;
+            for (int i = 0; i < S; ++i) {
+                h[i].x = sys[i].d_x;
+                h[i].q = sys[i].d_q;
+                h[i].phi = sys[i].d_phi;
+                h[i].force = sys[i].d_force;
+                h[i].N = sys[i].N;
+                h[i].off = (int)off;
+                h[i].blk = (int)blk;
+                h[i].pad = 0;
+                off += sys[i].N;
+                blk += (sys[i].N + 255) / 256;
+            }
+            SE_HIP(hipMemcpyAsync(pl->d_sys, h, S * sizeof(SysDesc), hipMemcpyHostToDevice, pl->stream));
+            SE_HIP(hipEventRecord(pl->h_ev[hb], pl->stream));
+            if (pl->esize == 4)
+                run_spread<float>(pl, (int)total, nullptr, nullptr, S, (int)blocks);
+            else
+                run_spread<double>(pl, (int)total, nullptr, nullptr, S, (int)blocks);
+        }
+        run_kspace(pl);
+        if (want_phi || want_force) {
+            if (pl->esize == 4)
+                run_interp<float>(pl, phi_mode, force_mode);
+            else
+                run_interp<double>(pl, phi_mode, force_mode);
+        }
+    });
 }
 
 int se_solve_graph(se_plan* pl, int N, const double* x, const double* q, double* phi, double* force)
@@ -653,7 +886,7 @@ int se_plan_info(se_plan* pl, int out[8])
         SE_HIP(hipStreamSynchronize(pl->stream));
         int ni = 0, bad = 0;
         SE_HIP(hipMemcpy(&ni, pl->n_items, sizeof(int), hipMemcpyDeviceToHost));
-        SE_HIP(hipMemcpy(&bad, pl->count + pl->tl.nbins, sizeof(int), hipMemcpyDeviceToHost));
+        SE_HIP(hipMemcpy(&bad, pl->count + (long)pl->last_S * pl->tl.nbins, sizeof(int), hipMemcpyDeviceToHost));
         out[4] = ni;
         out[5] = bad;
         out[6] = pl->tl.S;

@@ -81,6 +81,26 @@ struct alignas(16) Rec {
     int orig; // index in the caller's arrays
 };
 
+// One system of a grouped solve (several independent systems of one geometry in one
+// pass).  Flat particle index n of the group = off + local index; the particle kernels
+// run ceil(N/256) blocks per system starting at block `blk`.  Global tile index =
+// system * ntiles + tile, so sort keys, work items and partials of all systems share one
+// set of arrays; grids and spectra are stacked [system][x][y][z].
+struct SysDesc {
+    const double* x;   // SoA [3N]
+    const double* q;   // [N]
+    double* phi;       // [N] or null
+    double* force;     // SoA [3N] or null
+    int N, off, blk, pad;
+};
+
+// Output stores of the gathers.  Output pointers of a grouped solve are read from the
+// descriptors, so the compiler cannot tell their address space and would emit flat stores;
+// flat stores count against the LDS wait counter and turn the gathers' partial LDS waits
+// into full ones.  The outputs are device memory: store through global pointers.
+typedef __attribute__((address_space(1))) double global_double;
+__device__ __forceinline__ void store_out(double* p, long i, double v) { ((global_double*)p)[i] = v; }
+
 // Parameters of the reference-layout (legacy) kernels: SE_FGG_params + periodicity.
 struct LParams {
     int N, P, P_half, dims[3], npdims[3];
@@ -104,36 +124,48 @@ template <typename T>
 void launch_scatter(const double* x, const double* q, int N, const Geom& g, const Tiling& tl,
                     const int* key, const int* rank, const int* row_start, Rec<T>* rec,
                     hipStream_t s);
+// grouped forms: `blocks` = sum over systems of ceil(N/256); discard bucket S * nbins
+template <typename T>
+void launch_bin_keys_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, int* key,
+                           int* rank, int* count, hipStream_t s);
+template <typename T>
+void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, const int* key,
+                          const int* rank, const int* row_start, Rec<T>* rec, hipStream_t s);
 // pencil kernels (se_pencil.hip): FP32 only; return false if P has no compiled pencil kernel
 bool pencil_supported(int P);
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int* item_end,
                           const int* n_items, const Geom& g, const Tiling& tl, float* partials,
                           hipStream_t s);
+// sys != null: grouped solve, the outputs and N come from sys[tile / ntiles] and the
+// grids are stacked; phi/force then only select the output mode
 bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg,
                           const int* item_end, const int* n_items, const float* grid,
                           const Geom& g, const Tiling& tl, int N, double* phi, double* force,
-                          hipStream_t s);
+                          const SysDesc* sys, hipStream_t s);
 template <typename T>
 void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const Geom& g,
                          const Tiling& tl, T* partials, hipStream_t s);
-// merge segment tables: x and y share (seg, nseg), z has (segz, nsegz); both strides tl.nseg
+// merge segment tables: x and y share (seg, nseg), z has (segz, nsegz); both strides tl.nseg;
+// nsys stacked systems (tile index system * ntiles + tile)
 template <typename T>
 void launch_merge(const T* partials, const int* chunk_start, const int4* seg, const int* nseg, const int4* segz,
-                  const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid, hipStream_t s);
+                  const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid, int nsys,
+                  hipStream_t s);
 template <typename T>
 void launch_interp_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const T* grid, const Geom& g,
-                         const Tiling& tl, int N, double* phi, double* force, hipStream_t s);
+                         const Tiling& tl, int N, double* phi, double* force, const SysDesc* sys,
+                         hipStream_t s);
 // SE3P Green's function, separable tables: tab = [k_x^2 (M), e_x (M), k_z^2 (Mz), e_z (Mz), 4 pi/(M M Mz)]
 template <typename T>
-void launch_green(void* spec, int M, int Mz, const double* tab, hipStream_t s);
+void launch_green(void* spec, int M, int Mz, const double* tab, int nsys, hipStream_t s);
 void launch_green_tables(double* tab, int M, int Mz, double L, double Lz, double scalar, hipStream_t s);
 // SE2P Green's function, full half-spectrum table (see se_kernels.hip)
 void launch_green2p_table(double* tab, int M, int Mz, double L, double h, double R, double scalar,
                           hipStream_t s);
 template <typename T>
-void launch_green_full(void* spec, long n, const double* tab, hipStream_t s);
+void launch_green_full(void* spec, long n, const double* tab, int nsys, hipStream_t s);
 void launch_energy(const double* phi, const double* q, int N, double* out, hipStream_t s);
 
 // legacy (FP64, reference layout)

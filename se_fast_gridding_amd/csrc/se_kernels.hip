@@ -232,6 +232,79 @@ void launch_scatter(const double* x, const double* q, int N, const Geom& g, cons
                        row_start, rec);
 }
 
+// Grouped bucketing: block b belongs to the last system whose first block is <= b
+// (systems without particles own no block), found by a wave-uniform binary search.
+__device__ __forceinline__ int block_system(const SysDesc* __restrict__ sys, int S, int b)
+{
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sys[mid].blk <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ void k_bin_keys_group(const SysDesc* __restrict__ sys, int S, Geom g, Tiling tl, int* __restrict__ key,
+                                 int* __restrict__ rank, int* __restrict__ count)
+{
+    const int si = block_system(sys, S, blockIdx.x);
+    const SysDesc d = sys[si];
+    const int n = (blockIdx.x - d.blk) * 256 + threadIdx.x;
+    if (n >= d.N) return;
+    bool ok0, ok1, ok2;
+    double t;
+    const int w0 = window_start(d.x[n], g, t, ok0);
+    const int w1 = window_start(d.x[n + (long)d.N], g, t, ok1);
+    const int w2 = window_start_z(d.x[n + 2l * d.N], g, t, ok2);
+    const int k = (ok0 && ok1 && ok2) ? si * tl.nbins + row_key(w0, w1, w2, tl) : S * tl.nbins;
+    key[d.off + n] = k;
+    rank[d.off + n] = atomicAdd(&count[k], 1);
+}
+
+template <typename T>
+void launch_bin_keys_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, int* key,
+                           int* rank, int* count, hipStream_t s)
+{
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(k_bin_keys_group<T>, dim3(blocks), dim3(256), 0, s, sys, S, g, tl, key, rank, count);
+}
+
+template <typename T>
+__global__ void k_scatter_group(const SysDesc* __restrict__ sys, int S, Geom g, Tiling tl,
+                                const int* __restrict__ key, const int* __restrict__ rank,
+                                const int* __restrict__ row_start, Rec<T>* __restrict__ rec)
+{
+    const int si = block_system(sys, S, blockIdx.x);
+    const SysDesc d = sys[si];
+    const int n = (blockIdx.x - d.blk) * 256 + threadIdx.x;
+    if (n >= d.N) return;
+    bool ok;
+    double t0, t1, t2;
+    const int w0 = window_start(d.x[n], g, t0, ok);
+    const int w1 = window_start(d.x[n + (long)d.N], g, t1, ok);
+    const int w2 = window_start_z(d.x[n + 2l * d.N], g, t2, ok);
+    const int f = d.off + n, k = key[f];
+    Rec<T> r;
+    r.t[0] = (T)t0;
+    r.t[1] = (T)t1;
+    r.t[2] = (T)t2;
+    r.q = (T)d.q[n];
+    r.lo = (w0 % tl.TS) | ((w1 % tl.TS) << 8) | ((w2 % tl.TS) << 16);
+    r.orig = n;   // index inside the system; the gather finds the system from the tile
+    rec[row_start[k] + rank[f]] = r;
+}
+
+template <typename T>
+void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, const int* key,
+                          const int* rank, const int* row_start, Rec<T>* rec, hipStream_t s)
+{
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(k_scatter_group<T>, dim3(blocks), dim3(256), 0, s, sys, S, g, tl, key, rank, row_start,
+                       rec);
+}
+
 // ================================================================================
 // Tiled spread (generic P, FP32 or FP64).  One work item = up to `chunk` particles
 // of one tile.  The work-group zeroes an S^3 FP64 LDS sub-grid, each wave takes
@@ -385,6 +458,10 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
     // columns congruent to it mod 8.  Neighbours in z read the two ends of the same
     // z-rows of a partial, which then stay in that XCD's L2.
     const int NT = tl.NT, NTz = tl.NTz, TS = tl.TS, S = tl.S, M = g.M, Mz = g.Mz;
+    // stacked systems of a grouped solve: system blockIdx.y owns tiles [sys*ntiles, ...)
+    // and grid [sys][x][y][z]
+    chunk_start += (long)blockIdx.y * tl.ntiles;
+    grid += (long)blockIdx.y * M * M * Mz;
     const int xcd = blockIdx.x % 8, k = blockIdx.x / 8;
     const int col = xcd + 8 * (k / NTz), tz = k % NTz;
     if (col >= NT * NT) return;   // uniform: the last round of columns is partial
@@ -432,11 +509,12 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
 
 template <typename T>
 void launch_merge(const T* partials, const int* chunk_start, const int4* seg, const int* nseg, const int4* segz,
-                  const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid, hipStream_t s)
+                  const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid, int nsys, hipStream_t s)
 {
+    if (nsys <= 0) return;
     const int V = tl.TS * tl.TS * tl.TS;
     const int cols8 = (tl.NT * tl.NT + 7) / 8;
-    dim3 gr(8 * cols8 * tl.NTz), bl(256);
+    dim3 gr(8 * cols8 * tl.NTz, nsys), bl(256);
 #define SE_MG(E)                                                                                              \
     hipLaunchKernelGGL((k_merge_tiles<T, E>), gr, bl, 0, s, partials, chunk_start, seg, nseg, segz, nsegz,    \
                        sstride, g, tl, grid)
@@ -465,7 +543,7 @@ template <typename T, int PT, int UM, bool POT, bool FORCE>
 __global__ __launch_bounds__(512) void k_interp_tiles(
     const Rec<T>* __restrict__ rec, const int* __restrict__ item_tile, const int* __restrict__ item_beg,
     const int* __restrict__ item_end, const int* __restrict__ n_items, const T* __restrict__ grid, Geom g,
-    Tiling tl, int N, double* __restrict__ phi, double* __restrict__ force)
+    Tiling tl, int N, double* __restrict__ phi, double* __restrict__ force, const SysDesc* __restrict__ sys)
 {
     const int item = blockIdx.x;
     if (item >= *n_items) return;
@@ -475,7 +553,15 @@ __global__ __launch_bounds__(512) void k_interp_tiles(
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int M = g.M, Mz = g.Mz, S = tl.S, NT = tl.NT, NTz = tl.NTz;
     {
-        const int tile = item_tile[item];
+        int tile = item_tile[item];
+        if (sys) {   // grouped solve: the item's system, its outputs and its grid
+            const int si = tile / tl.ntiles;
+            tile -= si * tl.ntiles;
+            grid += (long)si * M * M * Mz;
+            phi = sys[si].phi;
+            force = sys[si].force;
+            N = sys[si].N;
+        }
         const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
         for (int e = tid; e < tl.part_elems; e += nthr) {
             int iz = e % S, r2 = e / S, iy = r2 % S, ix = r2 / S;
@@ -553,12 +639,12 @@ __global__ __launch_bounds__(512) void k_interp_tiles(
         }
         if (lane == 0) {
             const int o = r.orig;
-            if (POT) phi[o] = h3 * sp;
+            if (POT) store_out(phi, o, h3 * sp);
             if (FORCE) {
                 const double qh = h3 * (double)r.q;
-                force[o] = qh * s0;
-                force[o + (long)N] = qh * s1;
-                force[o + 2l * N] = qh * s2;
+                store_out(force, o, qh * s0);
+                store_out(force, o + (long)N, qh * s1);
+                store_out(force, o + 2l * N, qh * s2);
             }
         }
     }
@@ -567,7 +653,7 @@ __global__ __launch_bounds__(512) void k_interp_tiles(
 template <typename T, int PT, int UM>
 static void interp_dispatch_mode(const Rec<T>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                                  const int* n_items, const T* grid, const Geom& g, const Tiling& tl, int N,
-                                 double* phi, double* force, hipStream_t s)
+                                 double* phi, double* force, const SysDesc* sys, hipStream_t s)
 {
     size_t lds = (size_t)tl.sub_elems * sizeof(T);
     dim3 gr(tl.max_items), bl(tl.block);
@@ -575,7 +661,7 @@ static void interp_dispatch_mode(const Rec<T>* rec, const int* item_tile, const 
     do {                                                                                                       \
         allow_lds((const void*)k_interp_tiles<T, PT, UM, POT, FORCE>, lds);                                    \
         hipLaunchKernelGGL((k_interp_tiles<T, PT, UM, POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg,  \
-                           item_end, n_items, grid, g, tl, N, phi, force);                                     \
+                           item_end, n_items, grid, g, tl, N, phi, force, sys);                                \
     } while (0)
     if (phi && force)
         SE_GATHER(true, true);
@@ -589,10 +675,10 @@ static void interp_dispatch_mode(const Rec<T>* rec, const int* item_tile, const 
 template <typename T>
 void launch_interp_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                          const int* n_items, const T* grid, const Geom& g, const Tiling& tl, int N, double* phi,
-                         double* force, hipStream_t s)
+                         double* force, const SysDesc* sys, hipStream_t s)
 {
 #define SE_GD(PT, UM) interp_dispatch_mode<T, PT, UM>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, \
-                                                     phi, force, s)
+                                                     phi, force, sys, s)
     const int U = (g.P * g.P + 63) / 64;
     switch (g.P) {
     case 8: SE_GD(8, 1); break;
@@ -653,6 +739,7 @@ __global__ void k_green(T* __restrict__ spec, int M, int Mz, const double* __res
     const int Mh = Mz / 2 + 1;
     long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (gid >= (long)M * M * Mh) return;
+    spec += 2l * blockIdx.y * M * M * Mh;   // stacked spectra of a grouped solve
     const int kc = (int)(gid % Mh);
     const long r = gid / Mh;
     const int j = (int)(r % M), i = (int)(r / M);
@@ -665,10 +752,11 @@ __global__ void k_green(T* __restrict__ spec, int M, int Mz, const double* __res
 }
 
 template <typename T>
-void launch_green(void* spec, int M, int Mz, const double* tab, hipStream_t s)
+void launch_green(void* spec, int M, int Mz, const double* tab, int nsys, hipStream_t s)
 {
     long n = (long)M * M * (Mz / 2 + 1);
-    hipLaunchKernelGGL(k_green<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, (T*)spec, M, Mz, tab);
+    if (nsys <= 0) return;
+    hipLaunchKernelGGL(k_green<T>, dim3(cdiv(n, 256), nsys), dim3(256), 0, s, (T*)spec, M, Mz, tab);
 }
 
 // SE2P Green's function on the half spectrum of the M x M x Mz grid, z zero-padded.
@@ -716,15 +804,17 @@ __global__ void k_green_full(T* __restrict__ spec, long n, const double* __restr
 {
     long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (gid >= n) return;
+    spec += 2l * blockIdx.y * n;   // stacked spectra of a grouped solve
     const double z = tab[gid];
     spec[2 * gid] = (T)((double)spec[2 * gid] * z);
     spec[2 * gid + 1] = (T)((double)spec[2 * gid + 1] * z);
 }
 
 template <typename T>
-void launch_green_full(void* spec, long n, const double* tab, hipStream_t s)
+void launch_green_full(void* spec, long n, const double* tab, int nsys, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_green_full<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, (T*)spec, n, tab);
+    if (nsys <= 0) return;
+    hipLaunchKernelGGL(k_green_full<T>, dim3(cdiv(n, 256), nsys), dim3(256), 0, s, (T*)spec, n, tab);
 }
 
 // E = 1/2 sum phi q  (SE_fgg.c:5934-5946), FP64 block tree + one atomic per block.
@@ -1142,15 +1232,19 @@ void launch_reorder_gather(const double* xin, const double* qin, const int* perm
                                      hipStream_t);                                                               \
     template void launch_scatter<T>(const double*, const double*, int, const Geom&, const Tiling&, const int*,  \
                                     const int*, const int*, Rec<T>*, hipStream_t);                               \
+    template void launch_bin_keys_group<T>(const SysDesc*, int, int, const Geom&, const Tiling&, int*, int*,    \
+                                           int*, hipStream_t);                                                   \
+    template void launch_scatter_group<T>(const SysDesc*, int, int, const Geom&, const Tiling&, const int*,     \
+                                          const int*, const int*, Rec<T>*, hipStream_t);                         \
     template void launch_spread_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \
                                          const Geom&, const Tiling&, T*, hipStream_t);                           \
     template void launch_merge<T>(const T*, const int*, const int4*, const int*, const int4*, const int*, int,  \
-                                  const Geom&, const Tiling&, T*, hipStream_t);                                  \
+                                  const Geom&, const Tiling&, T*, int, hipStream_t);                             \
     template void launch_interp_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \
                                          const T*, const Geom&, const Tiling&, int, double*, double*,            \
-                                         hipStream_t);                                                           \
-    template void launch_green<T>(void*, int, int, const double*, hipStream_t);                                \
-    template void launch_green_full<T>(void*, long, const double*, hipStream_t);
+                                         const SysDesc*, hipStream_t);                                           \
+    template void launch_green<T>(void*, int, int, const double*, int, hipStream_t);                           \
+    template void launch_green_full<T>(void*, long, const double*, int, hipStream_t);
 SE_INST(float)
 SE_INST(double)
 

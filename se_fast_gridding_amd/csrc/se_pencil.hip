@@ -348,7 +348,8 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
                                                        const int* __restrict__ item_end,
                                                        const int* __restrict__ n_items,
                                                        const float* __restrict__ grid, Geom g, Tiling tl, int N,
-                                                       double* __restrict__ phi, double* __restrict__ force)
+                                                       double* __restrict__ phi, double* __restrict__ force,
+                                                       const SysDesc* __restrict__ sys)
 {
     constexpr int W = P + 1;
     constexpr int R = (W * W + 63) / 64;
@@ -359,7 +360,15 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
     const int tid = threadIdx.x, nthr = blockDim.x;
     {
-        const int tile = item_tile[item];
+        int tile = item_tile[item];
+        if (sys) {   // grouped solve: the item's system, its outputs and its grid
+            const int si = tile / tl.ntiles;
+            tile -= si * tl.ntiles;
+            grid += (long)si * M * M * Mz;
+            phi = sys[si].phi;
+            force = sys[si].force;
+            N = sys[si].N;
+        }
         const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
         for (int e = tid; e < S3; e += nthr) {
             const int iz = e % S, r2 = e / S, ix = r2 % S, iy = r2 / S;
@@ -435,12 +444,12 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
             s2 = wave_sum_dpp(s2);
         }
         if (lane == 0) {
-            if (POT) phi[o] = h3 * (double)sp;
+            if (POT) store_out(phi, o, h3 * (double)sp);
             if (FORCE) {
                 const double qh = h3 * (double)q;
-                force[o] = qh * (double)s0;
-                force[o + (long)N] = qh * (double)s1;
-                force[o + 2l * N] = qh * (double)s2;
+                store_out(force, o, qh * (double)s0);
+                store_out(force, o + (long)N, qh * (double)s1);
+                store_out(force, o + 2l * N, qh * (double)s2);
             }
         }
     };
@@ -550,7 +559,8 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int*
 template <int PV>
 static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg,
                                 const int* item_end, const int* n_items, const float* grid, const Geom& g,
-                                const Tiling& tl, int N, double* phi, double* force, hipStream_t s)
+                                const Tiling& tl, int N, double* phi, double* force, const SysDesc* sys,
+                                hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
     dim3 gr(tl.max_items), bl(tl.block);
@@ -558,7 +568,7 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_pencil<PV, POT, FORCE>, lds);                                       \
         hipLaunchKernelGGL((k_interp_pencil<PV, POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg,       \
-                           item_end, n_items, grid, g, tl, N, phi, force);                                    \
+                           item_end, n_items, grid, g, tl, N, phi, force, sys);                               \
     } while (0)
     if (phi && force)
         SE_GP(true, true);
@@ -571,14 +581,16 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
 
 bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
-                          double* phi, double* force, hipStream_t s)
+                          double* phi, double* force, const SysDesc* sys, hipStream_t s)
 {
+#define SE_GPM(PV) interp_pencil_modes<PV>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys, s)
     switch (g.P) {
-    case 8: interp_pencil_modes<8>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, s); return true;
-    case 11: interp_pencil_modes<11>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, s); return true;
-    case 16: interp_pencil_modes<16>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, s); return true;
+    case 8: SE_GPM(8); return true;
+    case 11: SE_GPM(11); return true;
+    case 16: SE_GPM(16); return true;
     default: return false;
     }
+#undef SE_GPM
 }
 
 }  // namespace se
