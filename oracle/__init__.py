@@ -89,6 +89,15 @@ def lib():
         L.or_direct3p_sf.argtypes = [_dp, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, C.c_double, C.c_int]
         L.or_direct2p_fd.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, _dp, C.c_double, C.c_int]
         L.or_direct2p_k0.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, _dp, C.c_double]
+        L.or_direct1p_fd.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, _dp, C.c_double, C.c_int]
+        L.or_direct1p_k0.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, _dp, C.c_double]
+        L.or_direct_real.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, C.c_int, _dp, C.c_double, C.c_int,
+                                     C.c_double]
+        L.or_direct_self.argtypes = [_dp, _dp, C.c_int, C.c_double]
+        L.or_k0inc.argtypes = [C.c_double, C.c_double]
+        L.or_k0inc.restype = C.c_double
+        L.or_ein.argtypes = [C.c_double]
+        L.or_ein.restype = C.c_double
         L.or_energy.argtypes = [_dp, _dp, C.c_int]
         L.or_energy.restype = C.c_double
         L.or_init_params.argtypes = [C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_double), C.c_double,
@@ -243,6 +252,52 @@ def direct2p(x, q, box, xi, layers, n_targets=None, with_k0=True):
         lib().or_direct2p_k0(k0, _f64(x), q, N, Nt, box, xi)
         phi = phi + k0
     return phi
+
+
+def direct1p(x, q, box, xi, layers, n_targets=None, with_k0=True):
+    """SE1P_direct_fd (+ SE1P_direct_k0), SE_direct.c:90-176: the SE1P k-space sum,
+    periodic in z with period box[2], free in x and y."""
+    q = _f64(q)
+    N = len(q)
+    Nt = N if n_targets is None else n_targets
+    box = _f64(np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)))
+    phi = np.zeros(Nt)
+    lib().or_direct1p_fd(phi, _f64(x), q, N, Nt, box, xi, layers)
+    if with_k0:
+        k0 = np.zeros(Nt)
+        lib().or_direct1p_k0(k0, _f64(x), q, N, Nt, box, xi)
+        phi = phi + k0
+    return phi
+
+
+def direct_real(per, x, q, box, xi, layers, rc=0.0, n_targets=None):
+    """Real-space Ewald sum over the images of the `per` periodic directions
+    (SE3P_direct_real[_rc], SE2P_direct_real[_rc], SE1P_direct_real[_rc])."""
+    q = _f64(q)
+    N = len(q)
+    Nt = N if n_targets is None else n_targets
+    box = _f64(np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)))
+    phi = np.zeros(Nt)
+    lib().or_direct_real(per, phi, _f64(x), q, N, Nt, box, xi, layers, rc)
+    return phi
+
+
+def direct_self(q, xi):
+    """-2 xi/sqrt(pi) q (SE3P/SE2P/SE1P_direct_self)."""
+    q = _f64(q)
+    phi = np.zeros(len(q))
+    lib().or_direct_self(phi, q, len(q), xi)
+    return phi
+
+
+def k0inc(a, b):
+    """Incomplete modified Bessel function int_0^1 t^-1 exp(-b t - a/t) dt."""
+    return lib().or_k0inc(a, b)
+
+
+def ein(x):
+    """Ein(x) = gamma + ln x + E1(x)."""
+    return lib().or_ein(x)
 
 
 def energy(phi, q):

@@ -536,6 +536,196 @@ void or_direct2p_k0(double* phi, const double* x, const double* q, int N, int Nt
     }
 }
 
+/* ---------------- SE1P direct sums (periodic in z, free in x and y) ---------------- */
+
+/* Gauss-Legendre rule on [a, b]: Newton iteration on the three-term Legendre
+ * recurrence from the Chebyshev guesses cos(pi (i - 1/4)/(n + 1/2)) (textbook). */
+static void or_gauss_legendre(int n, double a, double b, double* x, double* w)
+{
+    for (int i = 0; i < (n + 1) / 2; i++) {
+        double z = cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1;
+        for (int it = 0; it < 100; it++) {
+            double p0 = 1, p1 = z;
+            for (int k = 2; k <= n; k++) {
+                double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1);
+            double dz = p1 / dp;
+            z -= dz;
+            if (fabs(dz) < 1e-16) {
+                /* one more update of the derivative at the converged node */
+                p0 = 1; p1 = z;
+                for (int k = 2; k <= n; k++) {
+                    double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+                    p0 = p1;
+                    p1 = p2;
+                }
+                dp = n * (z * p1 - p0) / (z * z - 1);
+                break;
+            }
+        }
+        double wi = 2.0 / ((1 - z * z) * dp * dp);
+        x[i] = 0.5 * (a + b) - 0.5 * (b - a) * z;
+        x[n - 1 - i] = 0.5 * (a + b) + 0.5 * (b - a) * z;
+        w[i] = w[n - 1 - i] = 0.5 * (b - a) * wi;
+    }
+}
+
+/* Integrand of the incomplete Bessel function after t = e^{-y}:
+ * int_0^1 t^{-1} e^{-b t - a/t} dt = int_0^inf exp(-b e^{-y} - a e^{y}) dy. */
+static double or_k0inc_f(double a, double b, double y) { return exp(-b * exp(-y) - a * exp(y)); }
+
+static double or_gx16[16], or_gw16[16];
+
+/* the 16-point rule is built once, at load time (the direct sums call it from threads) */
+__attribute__((constructor)) static void or_gl16_init(void) { or_gauss_legendre(16, -1.0, 1.0, or_gx16, or_gw16); }
+
+static double or_gl16(double a, double b, double l, double r)
+{
+    const double* gx = or_gx16;
+    const double* gw = or_gw16;
+    double s = 0, c = 0.5 * (l + r), hw = 0.5 * (r - l);
+    for (int i = 0; i < 16; i++) s += gw[i] * or_k0inc_f(a, b, c + hw * gx[i]);
+    return s * hw;
+}
+
+static double or_k0inc_adapt(double a, double b, double l, double r, double whole, double tol, int depth)
+{
+    double m = 0.5 * (l + r);
+    double lft = or_gl16(a, b, l, m), rgt = or_gl16(a, b, m, r);
+    if (depth > 40 || fabs(lft + rgt - whole) <= tol) return lft + rgt;
+    return or_k0inc_adapt(a, b, l, m, lft, tol, depth + 1) + or_k0inc_adapt(a, b, m, r, rgt, tol, depth + 1);
+}
+
+/* Incomplete modified Bessel function K0(a, b) = int_0^1 t^{-1} exp(-b t - a/t) dt
+ * (the integrand of mathint.c:387-395, IncompBesselK0_int, integrated here
+ * adaptively to ~1e-16 instead of by the reference's fixed two-panel rule).  a > 0. */
+double or_k0inc(double a, double b)
+{
+    if (a <= 0) return HUGE_VAL;
+    /* beyond y = Y the integrand is below e^{-745} */
+    double Y = log(745.0 / a);
+    if (Y <= 0) return 0.0;
+    /* coarse pass to set the absolute tolerance */
+    double coarse = 0;
+    int np = 64;
+    for (int i = 0; i < np; i++) coarse += or_gl16(a, b, Y * i / np, Y * (i + 1) / np);
+    double tol = 1e-17 * fabs(coarse) + 1e-300, acc = 0;
+    for (int i = 0; i < np; i++) {
+        double l = Y * i / np, r = Y * (i + 1) / np;
+        acc += or_k0inc_adapt(a, b, l, r, or_gl16(a, b, l, r), tol, 0);
+    }
+    return acc;
+}
+
+/* Ein(x) = gamma + ln x + E1(x) = int_0^x (1 - e^{-t})/t dt: the power series
+ * sum (-1)^{k+1} x^k/(k k!) for x < 2, otherwise gamma + ln x + E1(x) with E1 from
+ * its continued fraction (modified Lentz). */
+double or_ein(double x)
+{
+    const double euler = 0.57721566490153286061;
+    if (x <= 0) return 0.0;
+    if (x < 2.0) {
+        double term = x, s = x;
+        for (int k = 2; k < 200; k++) {
+            term *= -x / k;
+            double add = term / k;
+            s += add;
+            if (fabs(add) < 1e-18 * fabs(s)) break;
+        }
+        return s;
+    }
+    /* E1(x) = e^{-x} / (x + 1 - 1/(x + 3 - 4/(x + 5 - ...))) */
+    double b = x + 1.0, c = 1e300, d = 1.0 / b, hh = d;
+    for (int i = 1; i < 1000; i++) {
+        double an = -(double)i * i;
+        b += 2.0;
+        d = 1.0 / (an * d + b);
+        c = b + an / c;
+        double del = c * d;
+        hh *= del;
+        if (fabs(del - 1.0) < 1e-17) break;
+    }
+    return euler + log(x) + hh * exp(-x);
+}
+
+/* SE_direct.c:99-155 (SE1P_direct_fd): k_z != 0 part of the SE1P k-space sum,
+ *   phi_m = 1/L sum_n q_n sum_{j != 0} cos(k_j z_mn) K0inc(k_j^2/(4 xi^2), rho_mn^2 xi^2),
+ * k_j = 2 pi j / L (L = box[2]); j and -j are summed together. */
+void or_direct1p_fd(double* phi, const double* x, const double* q, int N, int Nt,
+                    const double box[3], double xi, int layers)
+{
+    const double L = box[2];
+#pragma omp parallel for schedule(dynamic)
+    for (int m = 0; m < Nt; m++) {
+        double acc = 0;
+        for (int n = 0; n < N; n++) {
+            double dx = x[m] - x[n], dy = x[m + N] - x[n + N], z = x[m + 2 * N] - x[n + 2 * N];
+            double b = (dx * dx + dy * dy) * xi * xi;
+            for (int j = 1; j <= layers; j++) {
+                double k = 2 * M_PI * j / L;
+                acc += 2.0 * q[n] * cos(k * z) * or_k0inc(k * k / (4 * xi * xi), b);
+            }
+        }
+        phi[m] = acc / L;
+    }
+}
+
+/* SE_direct.c:158-176 (SE1P_direct_k0): the k_z = 0 part,
+ *   phi_m = -1/L sum_{n != m} q_n (gamma + ln(rho^2 xi^2) + E1(rho^2 xi^2)). */
+void or_direct1p_k0(double* phi, const double* x, const double* q, int N, int Nt,
+                    const double box[3], double xi)
+{
+    for (int m = 0; m < Nt; m++) {
+        double acc = 0;
+        for (int n = 0; n < N; n++) {
+            if (n == m) continue;
+            double dx = x[m] - x[n], dy = x[m + N] - x[n + N];
+            acc -= q[n] * or_ein((dx * dx + dy * dy) * xi * xi);
+        }
+        phi[m] = acc / box[2];
+    }
+}
+
+/* ---------------- real-space and self terms (SE_direct.c) ---------------- */
+
+/* Real-space Ewald sum sum_n sum_images q_n erfc(xi r)/r over the periodic images of
+ * the `per` periodic directions (3: all, SE_direct.c:464-505; 2: x and y,
+ * :184-254; 1: z, :15-88), |image index| <= layers, the m == n unshifted pair left
+ * out; pairs with r > rc are skipped (rc <= 0: no cutoff).  Targets: the first Nt
+ * particles.  Output is assigned (the reference's SE3P form adds to phi, :502). */
+void or_direct_real(int per, double* phi, const double* x, const double* q, int N, int Nt,
+                    const double box[3], double xi, int layers, double rc)
+{
+    int lx = per >= 2 ? layers : 0, ly = per >= 2 ? layers : 0, lz = (per == 3 || per == 1) ? layers : 0;
+#pragma omp parallel for schedule(dynamic)
+    for (int m = 0; m < Nt; m++) {
+        double acc = 0;
+        for (int n = 0; n < N; n++) {
+            double r0 = x[m] - x[n], r1 = x[m + N] - x[n + N], r2 = x[m + 2 * N] - x[n + 2 * N];
+            for (int p0 = -lx; p0 <= lx; p0++)
+                for (int p1 = -ly; p1 <= ly; p1++)
+                    for (int p2 = -lz; p2 <= lz; p2++) {
+                        if (m == n && !p0 && !p1 && !p2) continue;
+                        double a = r0 + p0 * box[0], b = r1 + p1 * box[1], c = r2 + p2 * box[2];
+                        double r = sqrt(a * a + b * b + c * c);
+                        if (rc > 0 && r > rc) continue;
+                        acc += q[n] * erfc(xi * r) / r;
+                    }
+        }
+        phi[m] = acc;
+    }
+}
+
+/* Self term -2 xi/sqrt(pi) q_m (SE_direct.c:178-183, 340-345, 550-561). */
+void or_direct_self(double* phi, const double* q, int N, double xi)
+{
+    double c = 2 * xi / sqrt(M_PI);
+    for (int m = 0; m < N; m++) phi[m] = -c * q[m];
+}
+
 /* SE_fgg.c:5934-5946 */
 double or_energy(const double* phi, const double* q, int N)
 {

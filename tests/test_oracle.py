@@ -235,3 +235,100 @@ def test_init_params_rule(oracle):
     M, m, Q = O.init_params(21, 3.1814, 8.0, a[:100, 3])
     assert M == 108 and np.isclose(m, 0.95 * np.sqrt(21 * np.pi))
     assert np.isclose(Q, np.sum(a[:100, 3] ** 2))
+
+
+# ---- SE1P direct sums, real-space and self terms ---------------------------------
+
+def test_k0inc_identity_and_limits(oracle):
+    """K0inc(a, b) + K0inc(b, a) = 2 K0(2 sqrt(ab)) (the two halves of the complete
+    integral, split at t = 1 and mapped by t -> 1/t), K0inc(a, 0) = E1(a); the modified
+    Bessel K0 and E1 come from scipy, independent of the oracle."""
+    sp = pytest.importorskip("scipy.special")
+    for a, b in [(0.1, 0.2), (1.5, 0.01), (0.02, 30.0), (3.0, 3.0), (0.154, 128.0), (1e-3, 1e-3), (40.0, 0.5)]:
+        ref = 2.0 * sp.k0(2.0 * np.sqrt(a * b))
+        assert abs(oracle.k0inc(a, b) + oracle.k0inc(b, a) - ref) <= 1e-14 * ref, (a, b)
+    for a in (0.01, 0.5, 3.0, 30.0):
+        assert abs(oracle.k0inc(a, 0.0) / sp.exp1(a) - 1.0) < 1e-14
+
+
+def test_ein_closed_forms(oracle):
+    """Ein(x) = gamma + ln x + E1(x) (scipy E1) where that form does not cancel, and
+    the leading terms x - x^2/4 + x^3/18 - x^4/96 of its series for tiny x."""
+    sp = pytest.importorskip("scipy.special")
+    for x in (0.5, 1.9, 2.1, 5.0, 30.0, 300.0):
+        ref = np.euler_gamma + np.log(x) + sp.exp1(x)
+        assert abs(oracle.ein(x) / ref - 1.0) < 1e-14, x
+    for x in (1e-8, 1e-4):
+        assert abs(oracle.ein(x) - (x - x * x / 4 + x ** 3 / 18 - x ** 4 / 96)) < 1e-16 * x
+
+
+def _neutral(N, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.random((3, N)).reshape(-1)
+    q = rng.uniform(-1, 1, N)
+    return x, q - q.mean()
+
+
+@pytest.mark.parametrize("per", [3, 2, 1])
+def test_ewald_total_is_independent_of_xi(oracle, per):
+    """real + k-space (+ k = 0 term) + self is the Coulomb sum of the periodic system,
+    whatever the splitting parameter xi: comparing xi = 3 with xi = 5 pins the real-space
+    sum, the k-space sums (SE3P; SE2P and SE1P with their k = 0 terms) and the self term
+    together -- a dropped term, a wrong sign or prefactor moves the xi = 3 and xi = 5
+    totals apart by O(1)."""
+    O = oracle
+    x, q = _neutral(20, 3)
+    box = (1.0, 1.0, 1.0)
+
+    def total(xi):
+        kl = int(np.ceil(6 * xi / np.pi)) + 2
+        real = O.direct_real(per, x, q, box, xi, 4 if per < 3 else 3)
+        if per == 3:
+            k = O.direct3p(x, q, box, xi, kl)
+        elif per == 2:
+            k = O.direct2p(x, q, box, xi, kl)
+        else:
+            k = O.direct1p(x, q, box, xi, kl)
+        return real + k + O.direct_self(q, xi)
+
+    a, b = total(3.0), total(5.0)
+    assert np.max(np.abs(a - b)) < 1e-13 * np.max(np.abs(a))
+    # each part alone does depend on xi
+    assert np.max(np.abs(O.direct_real(per, x, q, box, 3.0, 3) - O.direct_real(per, x, q, box, 5.0, 3))) > 1e-2
+
+
+def test_direct_real_cutoff_and_self(oracle):
+    """The rc variant drops exactly the pairs beyond rc (brute force over pairs), and the
+    self term is -2 xi/sqrt(pi) q."""
+    O = oracle
+    x, q = _neutral(12, 4)
+    X = x.reshape(3, -1)
+    xi, rc = 4.0, 0.45
+    got = O.direct_real(3, x, q, (1.0, 1.0, 1.0), xi, 1, rc=rc)
+    from math import erfc, sqrt
+    for m in range(12):
+        acc = 0.0
+        for n in range(12):
+            for s in np.ndindex(3, 3, 3):
+                sh = np.array(s) - 1
+                if m == n and not sh.any():
+                    continue
+                r = np.linalg.norm(X[:, m] - X[:, n] + sh)
+                if r <= rc:
+                    acc += q[n] * erfc(xi * r) / r
+        assert abs(got[m] - acc) < 1e-14 * max(1.0, abs(acc))
+    assert np.allclose(O.direct_self(q, xi), -2 * xi / sqrt(np.pi) * q, rtol=1e-15, atol=0)
+
+
+def test_direct1p_k0_is_permutation_equivariant(oracle):
+    """The SE1P k = 0 term is a sum over pairs of a kernel of the pair distance: relabelling
+    the particles permutes its outputs."""
+    O = oracle
+    x, q = _neutral(10, 6)
+    box = (1.0, 1.0, 1.0)
+    k0 = O.direct1p(x, q, box, 4.0, 1, with_k0=True) - O.direct1p(x, q, box, 4.0, 1, with_k0=False)
+    X = x.reshape(3, -1)
+    perm = np.roll(np.arange(10), 3)
+    k0p = O.direct1p(np.ascontiguousarray(X[:, perm].reshape(-1)), q[perm], box, 4.0, 1, with_k0=True) - \
+        O.direct1p(np.ascontiguousarray(X[:, perm].reshape(-1)), q[perm], box, 4.0, 1, with_k0=False)
+    assert np.allclose(k0p, k0[perm], rtol=0, atol=1e-14)
