@@ -37,7 +37,11 @@ double SE_init_params(int*, int*, double*, double, double, double, const double*
 /*
  * spectral_ewald.c:20-22.  SE3P k-space potential (force_phi = phi[N]) or force
  * (force_phi = F[3N], the reference's (q/2) grad(phi) convention; phi_energy, if
- * not NULL, also receives phi).  Which quantity is computed is set by
+ * not NULL, also receives phi).  With se_fgg_set_periodicity(2) it solves SE2P instead
+ * (periodic in x and y with period box[0], particles in 0 <= z < box[2]), with
+ * se_fgg_set_periodicity(1) SE1P (periodic in z with period box[2], particles in
+ * [0, box[0])^2 in x and y); M counts grid points per period.  The reference driver
+ * is SE3P only.  Which quantity is computed is set by
  * se_set_quantity() (default potential), replacing the reference's -DPOTENTIAL /
  * -DFORCE.  Arithmetic is FP64 (the reference's precision) unless
  * se_set_precision(SE_FP32) was called.  Returns 0 on success, nonzero on a HIP

@@ -39,10 +39,13 @@ extern "C" {
 typedef struct se_plan se_plan;
 
 typedef struct {
-    int periodicity;   /* 3 (SE3P) or 2 (SE2P: periodic in x and y, free in z) */
-    int M;             /* grid points per side of the periodic grid (x and y; also z for SE3P) */
+    int periodicity;   /* 3 (SE3P), 2 (SE2P: periodic in x and y, free in z) or
+                          1 (SE1P: periodic in z, free in x and y) */
+    int M;             /* grid points per period of the periodic directions (x and y for SE2P,
+                          z for SE1P, all three for SE3P); h = box / M */
     int P;             /* Gaussian support, 1..32 */
-    double box;        /* side length L of the box (SE3P: cube; SE2P: the periodic square) */
+    double box;        /* period L (SE3P: the cube side; SE2P: the periodic square's side;
+                          SE1P: the period in z) */
     double xi;         /* Ewald parameter */
     double m;          /* Gaussian shape; 0 selects the driver's 0.95*sqrt(pi*P) */
     int precision;     /* SE_PREC_FP64 or SE_PREC_FP32 (spread, FFT and gather arithmetic) */
@@ -51,10 +54,12 @@ typedef struct {
     int chunk;         /* max particles per work item, 0 = automatic */
     int block;         /* threads per work-group for spread/gather, 0 = automatic */
     int flags;         /* SE_PLAN_* bits */
-    double slab;       /* SE2P: particles have 0 <= z < slab (ignored for SE3P) */
-    int Mz;            /* SE2P: z grid points of the zero-padded free direction, 0 = automatic
-                          (the smallest FFT-friendly size that holds the slab, the support and
-                          the kernel truncation radius); ignored for SE3P */
+    double slab;       /* SE2P: particles have 0 <= z < slab; SE1P: particles have
+                          0 <= x, y < slab (ignored for SE3P) */
+    int Mz;            /* grid points of the zero-padded free direction(s), 0 = automatic (the
+                          smallest FFT-friendly size that holds the particles' range, the support
+                          and the kernel truncation radius): z for SE2P, x and y for SE1P;
+                          ignored for SE3P */
 } se_plan_desc;
 
 /* Use the generic one-particle-per-wave kernels even where the FP32 register-ring
@@ -79,7 +84,19 @@ size_t se_plan_bytes_for(const se_plan_desc* desc);
  * counted as discarded.
  */
 
-/* Grid shape of the plan: dims = {M, M, Mz}; *z0 = z coordinate of grid index 0 in z. */
+/*
+ * SE1P: the potential and force are the SE1P Ewald k-space sum (SE_direct.c:90-176: the
+ * k_z != 0 terms plus the k_z = 0 term, whose constant follows the reference's
+ * gamma + ln(rho^2 xi^2) + E1(rho^2 xi^2) also for a net charge).  x and y are zero-padded
+ * to Mz points each and use the in-plane Coulomb kernel of each z mode truncated at a
+ * radius that covers the square's diagonal and the Gaussian supports; x and y grid index
+ * 0 sits at a + h/2 with a = -(P_half + 1) h (the reference's SE1P grid origin convention,
+ * SE_fgg.c:836-846).  Particles with x or y outside [0, slab) are left out and counted as
+ * discarded.
+ */
+
+/* Grid shape of the plan, [x][y][z] with z fastest: dims = {M, M, Mz} for SE3P and SE2P,
+ * {Mz, Mz, M} for SE1P; *z0 = z coordinate of grid index 0 in z (0 for SE3P and SE1P). */
 int se_plan_grid_shape(const se_plan* plan, int dims[3], double* z0);
 
 /* Full k-space solve.  phi and/or force may be NULL (not computed). */
@@ -168,10 +185,10 @@ const char* se_gpu_last_error(void);
 #define SE_BUSY 1
 
 typedef struct {
-    int periodicity;   /* 3 or 2 */
+    int periodicity;   /* 3, 2 or 1 */
     int M, P;
     double box, xi, m; /* m = 0: 0.95 sqrt(pi P) */
-    double slab;       /* SE2P slab height */
+    double slab;       /* SE2P slab height; SE1P side of the x-y square */
     int precision;     /* SE_PREC_FP64 / SE_PREC_FP32 */
 } se_job_shape;
 

@@ -235,12 +235,14 @@ def _ptr(t):
 
 
 class Plan:
-    """Device-resident SE3P / SE2P solver (include/se_gpu.h) driven with torch CUDA tensors.
+    """Device-resident SE3P / SE2P / SE1P solver (include/se_gpu.h) driven with torch CUDA tensors.
 
     x is (3N,) float64 SoA [x..., y..., z...], q is (N,) float64, outputs are float64,
     force SoA (3N,) in the reference's (q/2) grad(phi) convention (SE_fgg.c:2291-2307).
     periodicity=2 solves SE2P: periodic in x and y (period box), free in z with particles
-    in 0 <= z < slab; Mz (0 = automatic) is the zero-padded z grid.  max_systems > 1
+    in 0 <= z < slab; Mz (0 = automatic) is the zero-padded z grid.  periodicity=1 solves
+    SE1P: periodic in z (period box, M grid points), free in x and y with particles in
+    [0, slab)^2; Mz (0 = automatic) is then the zero-padded x-y grid side.  max_systems > 1
     makes a group plan (n_max then counts the particles of all systems together) for
     solve_group."""
 

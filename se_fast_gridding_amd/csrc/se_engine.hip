@@ -108,7 +108,7 @@ struct se_plan {
     Geom g{};
     Tiling tl{};
     double L = 1, xi = 1, m = 1, eta = 0;
-    double R2p = 0;   // SE2P kernel truncation radius
+    double R2p = 0;   // SE2P / SE1P kernel truncation radius
     hipStream_t stream = nullptr;
     int n_cap = 0;
     int last_N = -1;
@@ -216,8 +216,8 @@ int fft_size(int nmin)
 
 void derive(const se_plan_desc* d, Geom& g, double& eta, double& m, double& R2p)
 {
-    SE_REQUIRE(d->periodicity == 3 || d->periodicity == 2, SE_ERR_ARG,
-               "se_plan: periodicity must be 3 (SE3P) or 2 (SE2P)");
+    SE_REQUIRE(d->periodicity == 3 || d->periodicity == 2 || d->periodicity == 1, SE_ERR_ARG,
+               "se_plan: periodicity must be 3 (SE3P), 2 (SE2P) or 1 (SE1P)");
     SE_REQUIRE(d->M >= 1 && d->P >= 1 && d->P <= 32 && d->box > 0 && d->xi > 0, SE_ERR_ARG,
                "invalid plan description (need M>=1, 1<=P<=32, box>0, xi>0)");
     m = d->m > 0 ? d->m : 0.95 * std::sqrt(M_PI * d->P);
@@ -235,6 +235,30 @@ void derive(const se_plan_desc* d, Geom& g, double& eta, double& m, double& R2p)
     g.c = c;
     g.d = std::pow(c / M_PI, 1.5);
     g.per = d->periodicity;
+    g.x0 = 0.0;
+    g.xlo = -HUGE_VAL;
+    g.xhi = HUGE_VAL;
+    if (g.per == 1) {
+        // SE1P: z periodic with M points; x and y free, grid origin a + h/2 with
+        // a = -(P_half + 1) h in both, so every support window of a particle with
+        // 0 <= x, y < slab starts at index >= 0.  The in-plane kernel truncation radius R
+        // covers the diagonal of the charged and read grid points plus the Gaussian
+        // screening tail; the padded period holds R plus their span D.
+        SE_REQUIRE(d->slab > 0, SE_ERR_ARG, "SE1P plan needs slab > 0 (the side of the x-y square)");
+        g.Mz = d->M;
+        g.z0 = 0.0;
+        g.zlo = -HUGE_VAL;
+        g.zhi = HUGE_VAL;
+        g.x0 = -(double)(ph + 1) * h + 0.5 * h;
+        g.xlo = 0.0;
+        g.xhi = d->slab;
+        R2p = std::sqrt(2.0) * (d->slab + (double)(d->P + 1) * h) + 6.0 / d->xi;
+        const double D = d->slab + (double)(d->P + 2 * ph + 2) * h;
+        const int nmin = (int)std::ceil((R2p + D) / h) + 1;
+        g.M = d->Mz > 0 ? d->Mz : fft_size(nmin);
+        SE_REQUIRE(g.M >= nmin, SE_ERR_ARG, "SE1P x-y grid (Mz) too small for the square, support and kernel radius");
+        return;
+    }
     if (g.per == 3) {
         g.Mz = d->M;
         g.z0 = 0.0;
@@ -316,6 +340,58 @@ void build_merge_segments(se_plan* pl)
     upload(lz, pl->segz, pl->nsegz);
 }
 
+// Modified Bessel functions K0 and K1 from K_nu(x) = int_0^inf e^{-x cosh t} cosh(nu t) dt
+// by the trapezoidal rule, which converges like e^{-pi^2/dt} relative to e^{x} K_nu(x) for
+// this integrand (analytic in |Im t| < pi/2), so dt = pi^2/(x + 40) gives full FP64
+// accuracy; the sum runs until e^{-x cosh t} underflows.  Zero for x > 700 and for
+// x <= 0 (where K0 and K1 are infinite; the k_z = 0 mode does not use them).
+static void bessel_k01(double x, double& k0, double& k1)
+{
+    k0 = k1 = 0.0;
+    if (!(x > 0.0) || x > 700.0) return;
+    const double dt = std::min(0.1, M_PI * M_PI / (x + 40.0)) / 1.25;
+    const double tmax = std::acosh(std::max(1.0, 745.0 / x)) + dt;
+    double s0 = 0.5 * std::exp(-x), s1 = s0;
+    for (double t = dt; t <= tmax; t += dt) {
+        const double e = std::exp(-x * std::cosh(t));
+        s0 += e;
+        s1 += e * std::cosh(t);
+    }
+    k0 = s0 * dt;
+    k1 = s1 * dt;
+}
+
+// SE1P Green's function table: Bessel J0, J1 of |k_par| R per in-plane frequency pair and
+// K0, K1 of k_z R per z frequency on the host (FP64, libm j0/j1), the table on the device.
+// The k = 0 value is the truncated log kernel's -pi R^2 (2 ln R - 1) shifted by
+// -(M h)^2 (gamma + ln xi^2): the FFT convolution then reproduces the reference's k_z = 0
+// term -(1/L) sum_n q_n (gamma + ln(rho^2 xi^2) + E1(rho^2 xi^2)) (SE_direct.c:158-176)
+// exactly, also for systems with a net charge.
+static void build_green1p(se_plan* pl, double scalar)
+{
+    const int M = pl->g.M, Mz = pl->g.Mz, Mh = Mz / 2 + 1;
+    const double h = pl->g.h, R = pl->R2p, Lam = M * h, Lz = Mz * h;
+    std::vector<double> jt(2 * (size_t)M * M), kt(2 * (size_t)Mh);
+    auto freq = [](int i, int n, double period) {
+        const int pos = (n % 2 == 0) ? n / 2 : (n - 1) / 2 + 1;
+        return 2.0 * M_PI / period * (double)(i < pos ? i : i - n);
+    };
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) {
+            const double kx = freq(i, M, Lam), ky = freq(j, M, Lam), kr = std::sqrt(kx * kx + ky * ky) * R;
+            jt[(size_t)i * M + j] = ::j0(kr);
+            jt[(size_t)M * M + (size_t)i * M + j] = ::j1(kr);
+        }
+    for (int k = 0; k < Mh; ++k) bessel_k01(2.0 * M_PI / Lz * k * R, kt[k], kt[Mh + k]);
+    const double euler = 0.57721566490153286061;
+    const double c00 = -M_PI * R * R * (2.0 * std::log(R) - 1.0) - Lam * Lam * (euler + std::log(pl->xi * pl->xi));
+    double* dj = (double*)pl->alloc(jt.size() * sizeof(double));
+    double* dk = (double*)pl->alloc(kt.size() * sizeof(double));
+    SE_HIP(hipMemcpy(dj, jt.data(), jt.size() * sizeof(double), hipMemcpyHostToDevice));
+    SE_HIP(hipMemcpy(dk, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice));
+    launch_green1p_table(pl->gtab, dj, dk, M, Mz, h, Lz, R, scalar, c00, pl->stream);
+}
+
 size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
 {
     Geom g;
@@ -323,9 +399,9 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
     double R2p = 0;
     derive(d, g, eta, m, R2p);
     const int es = d->precision == SE_PREC_FP32 ? 4 : 8;
-    Tiling tl = choose_tiling(es, d->P, d->M, g.Mz, d->n_max, n_sys, d->tile, d->chunk, d->block, lds_max,
+    Tiling tl = choose_tiling(es, d->P, g.M, g.Mz, d->n_max, n_sys, d->tile, d->chunk, d->block, lds_max,
                               !(d->flags & SE_PLAN_GENERIC_KERNELS));
-    const size_t M = d->M, Mz = g.Mz, N = d->n_max, ns = n_sys;
+    const size_t M = g.M, Mz = g.Mz, N = d->n_max, ns = n_sys;
     size_t b = 0;
     b += 2 * N * 4;                                       // key, rank
     b += 2 * (size_t)(ns * tl.nbins + 1) * 4;             // count, row_start
@@ -336,7 +412,7 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
     b += ns * M * M * Mz * es;                            // grids
     b += ns * M * M * (Mz / 2 + 1) * 2 * es;              // half spectra
     b += ns * M * M * (Mz / 2 + 1) * 2 * es;              // FFT work (upper estimate)
-    if (g.per == 2) b += M * M * (Mz / 2 + 1) * 8;        // SE2P Green's function table
+    if (g.per != 3) b += M * M * (Mz / 2 + 1) * 8;        // SE2P / SE1P Green's function table
     if (ns > 1) b += ns * sizeof(SysDesc);                // descriptors
     return b;
 }
@@ -576,10 +652,10 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         pl->stream = (hipStream_t)stream;
         pl->n_cap = std::max(d->n_max, 1);
         const int lds_max = lds_limit();
-        pl->tl = choose_tiling(pl->esize, d->P, d->M, pl->g.Mz, pl->n_cap, n_sys, d->tile, d->chunk, d->block,
+        pl->tl = choose_tiling(pl->esize, d->P, pl->g.M, pl->g.Mz, pl->n_cap, n_sys, d->tile, d->chunk, d->block,
                                lds_max, !(d->flags & SE_PLAN_GENERIC_KERNELS));
         const Tiling& tl = pl->tl;
-        const size_t M = d->M, Mz = pl->g.Mz, N = pl->n_cap, ns = n_sys;
+        const size_t M = pl->g.M, Mz = pl->g.Mz, N = pl->n_cap, ns = n_sys;
         SE_REQUIRE(ns * tl.nbins < (1ul << 31) - 1 && ns * tl.ntiles * (size_t)tl.K < (1ul << 31) - 1, SE_ERR_ARG,
                    "too many systems for the sort keys of this geometry");
         pl->key = (int*)pl->alloc(N * 4);
@@ -612,9 +688,11 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         // Green's function: s = -(1-eta)/(4 xi^2)   (spectral_ewald.c:126)
         const double gs = -(1.0 - pl->eta) / (4.0 * d->xi * d->xi);
         if (pl->g.per == 3)
-            launch_green_tables(pl->gtab, d->M, (int)Mz, d->box, d->box, gs, pl->stream);
+            launch_green_tables(pl->gtab, (int)M, (int)Mz, d->box, d->box, gs, pl->stream);
+        else if (pl->g.per == 2)
+            launch_green2p_table(pl->gtab, (int)M, (int)Mz, d->box, pl->g.h, pl->R2p, gs, pl->stream);
         else
-            launch_green2p_table(pl->gtab, d->M, (int)Mz, d->box, pl->g.h, pl->R2p, gs, pl->stream);
+            build_green1p(pl, gs);
         // rocFFT R2C / C2R for one grid; a group plan also sizes the work area for its
         // largest batch (the batched plans themselves are made per group size on use)
         rocfft_init_once();

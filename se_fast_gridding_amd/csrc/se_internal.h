@@ -39,7 +39,10 @@ struct Error {
 // u = x/h in x and y and u = (z - z0)/h in z (SE_fgg.c:554-573 for SE3P, where Mz = M
 // and z0 = 0).  For SE2P (per == 2) the z grid is the free direction, zero-padded so
 // that no support window reaches its ends; z0 = a + h/2 is the reference's 2P grid
-// origin (SE_fgg.c:753-764) and particles outside [zlo, zhi) are left out.
+// origin (SE_fgg.c:753-764) and particles outside [zlo, zhi) are left out.  For SE1P
+// (per == 1) x and y are the free directions instead: u = (x - x0)/h in x and y with
+// x0 = a + h/2 (the reference's 1P grid origin, SE_fgg.c:836-846), both zero-padded to
+// M points, and particles outside [xlo, xhi) in x or y are left out; z is periodic.
 // ---------------------------------------------------------------------------------
 struct Geom {
     int M;        // grid points per side in x and y
@@ -48,9 +51,11 @@ struct Geom {
     int odd;
     double h, c, d;   // spacing, Gaussian exponent, normalisation (c/pi)^1.5
     int Mz;       // grid points in z
-    int per;      // 3: SE3P, 2: SE2P (z free)
+    int per;      // 3: SE3P, 2: SE2P (z free), 1: SE1P (x and y free)
     double z0;    // z coordinate of grid index 0 in z
     double zlo, zhi;  // SE2P slab: accepted particle z range
+    double x0;        // x and y coordinate of grid index 0 in x and y (SE1P; 0 otherwise)
+    double xlo, xhi;  // SE1P: accepted particle x and y range
 };
 
 // Tile decomposition used by the bucketed spread/gather.  A tile of TS^3 window
@@ -164,6 +169,11 @@ void launch_green_tables(double* tab, int M, int Mz, double L, double Lz, double
 // SE2P Green's function, full half-spectrum table (see se_kernels.hip)
 void launch_green2p_table(double* tab, int M, int Mz, double L, double h, double R, double scalar,
                           hipStream_t s);
+// SE1P Green's function, full half-spectrum table of the M x M (free) x Mz (periodic) grid;
+// jtab = [J0(|k_par| R), J1(|k_par| R)] per (x, y) frequency pair (M*M each), ktab =
+// [K0(k_z R), K1(k_z R)] per z frequency (Mz/2+1 each), c00 = the k = 0 value
+void launch_green1p_table(double* tab, const double* jtab, const double* ktab, int M, int Mz, double h, double Lz,
+                          double R, double scalar, double c00, hipStream_t s);
 template <typename T>
 void launch_green_full(void* spec, long n, const double* tab, int nsys, hipStream_t s);
 void launch_energy(const double* phi, const double* q, int N, double* out, hipStream_t s);

@@ -93,13 +93,16 @@ void launch_fill_bytes0(void* p, size_t bytes, hipStream_t s)
 // ================================================================================
 // Bucketing: particle -> (tile, pencil, row) of its window start (counting sort)
 // ================================================================================
+// x and y: offset by the grid origin x0 (0 unless SE1P); for SE1P the particle must lie
+// in [xlo, xhi) (the free directions' accepted range)
 __device__ __forceinline__ int window_start(double x, const Geom& g, double& t, bool& ok)
 {
-    double u = x / g.h;
-    ok = fabs(u) < 1.0e9;
+    const double xr = x - g.x0;
+    double u = xr / g.h;
+    ok = fabs(u) < 1.0e9 && (g.per != 1 || (x >= g.xlo && x < g.xhi));
     if (!ok) u = 0.0;
     int iota = g.odd ? (int)round(u) : (int)floor(u);
-    t = x - g.h * iota;
+    t = xr - g.h * iota;
     return imod(iota + g.first, g.M);
 }
 
@@ -797,6 +800,50 @@ void launch_green2p_table(double* tab, int M, int Mz, double L, double h, double
 {
     long n = (long)M * M * (Mz / 2 + 1);
     hipLaunchKernelGGL(k_green2p_table, dim3(cdiv(n, 256)), dim3(256), 0, s, tab, M, Mz, L, h, R, scalar);
+}
+
+// SE1P Green's function on the half spectrum of the M x M x Mz grid, x and y free and
+// zero-padded, z periodic (period Lz).  Per z mode k_z the in-plane kernel is the 2-D
+// Green's function of the mode, 2 K0(|k_z| rho) (k_z != 0) or -2 ln rho (k_z = 0),
+// truncated at rho <= R; its 2-D transform at k = |k_par| is exact:
+//   k_z != 0:  4 pi [1 + k R J1(kR) K0(k_z R) - k_z R J0(kR) K1(k_z R)] / (k^2 + k_z^2),
+//   k_z == 0:  -4 pi [R ln R J1(kR)/k + (J0(kR) - 1)/k^2],   c00 at k = 0,
+// (from d/drho [rho (k J1 K0 - k_z J0 K1)] = (k^2 + k_z^2) rho J0 K0 and
+// d/drho [rho ln rho J1/k + J0/k^2] = rho ln rho J0), times e^{scalar (k^2 + k_z^2)} /
+// (M M Mz).  With the padded period M h >= R + (span of the charged and read grid
+// points) the periodic FFT convolution equals the free one in x and y, and the potential
+// is the SE1P Ewald k-space sum (SE_direct.c:90-176).
+__global__ void k_green1p_table(double* __restrict__ tab, const double* __restrict__ jtab,
+                                const double* __restrict__ ktab, int M, int Mz, double h, double Lz, double R,
+                                double scalar, double c00)
+{
+    const int Mh = Mz / 2 + 1;
+    long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (gid >= (long)M * M * Mh) return;
+    const int kc = (int)(gid % Mh);
+    const long r = gid / Mh;
+    const int j = (int)(r % M), i = (int)(r / M);
+    const double kx = kfreq(i, M, (double)M * h), ky = kfreq(j, M, (double)M * h);
+    const double kz = 2.0 * M_PI / Lz * (double)kc;
+    const double kp2 = kx * kx + ky * ky, kp = sqrt(kp2), k2 = kp2 + kz * kz;
+    const double J0 = jtab[r], J1 = jtab[(long)M * M + r];
+    double G;
+    if (kc > 0) {
+        G = 4.0 * M_PI * (1.0 + kp * R * J1 * ktab[kc] - kz * R * J0 * ktab[Mh + kc]) / k2;
+    } else if (kp > 0.0) {
+        G = -4.0 * M_PI * (R * log(R) * J1 / kp + (J0 - 1.0) / kp2);
+    } else {
+        G = c00;
+    }
+    tab[gid] = G * exp(scalar * k2) / ((double)M * M * Mz);
+}
+
+void launch_green1p_table(double* tab, const double* jtab, const double* ktab, int M, int Mz, double h, double Lz,
+                          double R, double scalar, double c00, hipStream_t s)
+{
+    long n = (long)M * M * (Mz / 2 + 1);
+    hipLaunchKernelGGL(k_green1p_table, dim3(cdiv(n, 256)), dim3(256), 0, s, tab, jtab, ktab, M, Mz, h, Lz, R, scalar,
+                       c00);
 }
 
 template <typename T>

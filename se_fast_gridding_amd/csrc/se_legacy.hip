@@ -205,27 +205,35 @@ bool avx_dispatch_accumulates(const SE_FGG_params* p)
 }
 
 struct PlanKey {
-    int M, P, prec, cap;
-    double L, xi, m;
+    int M, P, prec, cap, per;
+    double L, xi, m, slab;
 };
 se_plan* g_plan = nullptr;
 PlanKey g_key{};
 
+// The driver's plan for the periodicity chosen with se_fgg_set_periodicity: SE3P in the
+// cube of side box[0] (the reference driver, spectral_ewald.c:20-182); SE2P periodic in
+// x and y with period box[0], particles in 0 <= z < box[2]; SE1P periodic in z with
+// period box[2], particles in [0, box[0])^2 in x and y.  M counts grid points per period.
 se_plan* driver_plan(const SE_opt& o, double xi, int N, int prec)
 {
-    PlanKey k{o.M, o.P, prec, std::max(N, 1), o.box[0], xi, o.m};
-    if (g_plan && g_key.M == k.M && g_key.P == k.P && g_key.prec == k.prec && g_key.L == k.L && g_key.xi == k.xi &&
-        g_key.m == k.m && g_key.cap >= N)
+    const int per = g_periodicity;
+    const double L = per == 1 ? o.box[2] : o.box[0];
+    const double slab = per == 2 ? o.box[2] : (per == 1 ? o.box[0] : 0.0);
+    PlanKey k{o.M, o.P, prec, std::max(N, 1), per, L, xi, o.m, slab};
+    if (g_plan && g_key.M == k.M && g_key.P == k.P && g_key.prec == k.prec && g_key.per == k.per && g_key.L == k.L &&
+        g_key.xi == k.xi && g_key.m == k.m && g_key.slab == k.slab && g_key.cap >= N)
         return g_plan;
     if (g_plan) {
         se_plan_destroy(g_plan);
         g_plan = nullptr;
     }
     se_plan_desc d{};
-    d.periodicity = 3;
+    d.periodicity = per;
+    d.slab = slab;
     d.M = o.M;
     d.P = o.P;
-    d.box = o.box[0];
+    d.box = L;
     d.xi = xi;
     d.m = o.m;
     d.precision = prec == SE_FP32 ? SE_PREC_FP32 : SE_PREC_FP64;

@@ -108,9 +108,15 @@ def test_plan_rejects_bad_descriptions_without_gpu_compute():
     h = C.c_void_p()
     d = se.se_plan_desc(3, 0, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0, 0, 0.0, 0)
     assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
-    d = se.se_plan_desc(1, 32, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0, 0, 1.0, 0)
+    d = se.se_plan_desc(4, 32, 16, 1.0, 1.0, 0.0, 1, 10, 0, 0, 0, 0, 1.0, 0)
     assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
     assert "periodicity" in se.last_error()
+    d = se.se_plan_desc(1, 32, 16, 1.0, 6.0, 0.0, 1, 10, 0, 0, 0, 0, 0.0, 0)   # SE1P without a square
+    assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
+    assert "slab" in se.last_error()
+    d = se.se_plan_desc(1, 32, 16, 1.0, 6.0, 0.0, 1, 10, 0, 0, 0, 0, 1.0, 64)  # x-y grid too small
+    assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
+    assert "Mz" in se.last_error()
     d = se.se_plan_desc(2, 32, 16, 1.0, 6.0, 0.0, 1, 10, 0, 0, 0, 0, 0.0, 0)   # SE2P without a slab
     assert lib.se_plan_create(C.byref(h), C.byref(d), None) == se.SE_ERR_ARG
     assert "slab" in se.last_error()

@@ -235,3 +235,43 @@ def test_spectral_ewald_atoms_fixture(oracle):
     assert lib.spectral_ewald(dp(x), dp(q), opt, xi, dp(phi), None) == 0
     ref = oracle.se3p(x, q, L, M.value, P, m.value, xi)
     assert rms_rel(phi, ref) < 1e-12
+
+
+@pytest.mark.parametrize("per", [2, 1])
+def test_spectral_ewald_driver_se2p_se1p(oracle, per):
+    """spectral_ewald after se_fgg_set_periodicity(2 | 1): the SE2P / SE1P k-space sums of
+    the reference's direct solver (SE_direct.c:90-176, 256-338) at the FGG error floor,
+    potential and force (F against the device plan's force for the same system)."""
+    import torch
+    lib = se.capi()
+    rng = np.random.default_rng(10 + per)
+    N, P, M, xi = 40, 21, 32, 6.0
+    x = rng.random(3 * N)
+    q = rng.uniform(-1, 1, N)
+    q -= q.mean()
+    opt = se.SE_opt()
+    opt.m = 0.95 * np.sqrt(np.pi * P)
+    opt.box[0] = opt.box[1] = opt.box[2] = 1.0
+    opt.P, opt.M, opt.N = P, M, N
+    phi = np.zeros(N)
+    F = np.zeros(3 * N)
+    lib.se_fgg_set_periodicity(per)
+    try:
+        lib.se_set_quantity(1)
+        assert lib.spectral_ewald(dp(x), dp(q), opt, xi, dp(phi), None) == 0
+        lib.se_set_quantity(2)
+        assert lib.spectral_ewald(dp(x), dp(q), opt, xi, dp(F), None) == 0
+    finally:
+        lib.se_set_quantity(1)
+        lib.se_fgg_set_periodicity(3)
+    if per == 2:
+        ref = oracle.direct2p(x, q, (1.0, 1.0, 1.0), xi, 14)
+    else:
+        ref = oracle.direct1p(x, q, (1.0, 1.0, 1.0), xi, int(np.ceil(2 * xi * np.sqrt(40.0) / (2 * np.pi))) + 1)
+    assert rms_rel(phi, ref) < 1e-11
+    plan = se.Plan(M, P, 1.0, xi, N, periodicity=per, slab=1.0)
+    Ft = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+    plan.solve(torch.tensor(x, device="cuda"), torch.tensor(q, device="cuda"), None, Ft)
+    torch.cuda.synchronize()
+    plan.close()
+    assert rms_rel(F, Ft.cpu().numpy()) < 1e-13
