@@ -160,7 +160,7 @@ def run(args):
     prec = se.PREC_FP32 if args.precision == "fp32" else se.PREC_FP64
     stream = torch.cuda.current_stream()
     plan = se.Plan(M, P, 1.0, xi, N, precision=prec, stream=stream, tile=args.tile, chunk=args.chunk,
-                   block=args.block, generic=args.generic)
+                   block=args.block, generic=args.generic, mfma=args.mfma)
 
     for _ in range(args.warmup):
         plan.solve(xt, qt, phi, F)
@@ -192,10 +192,11 @@ def run(args):
     spread_flops = 2.0 * N * P ** 3
     gather_flops = 4.0 * N * P ** 3
     kind = "pencil" if info["pencil"] else "tiles"
+    gname = "k_interp_mfma16" if (info["pencil"] and P == 16 and args.mfma) else "k_interp_" + kind
     if ms["spread_kernel"] >= ms["gather_kernel"]:
         kname, kflops, kms = "k_spread_" + kind, spread_flops, ms["spread_kernel"]
     else:
-        kname, kflops, kms = "k_interp_" + kind, gather_flops, ms["gather_kernel"]
+        kname, kflops, kms = gname, gather_flops, ms["gather_kernel"]
     achieved = kflops / (kms * 1e-3) / 1e12
     traffic = MEASURED_TRAFFIC.get((kname, N, P, M, args.precision, info["tile"]))
 
@@ -437,6 +438,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="max particles per work item (0 = default)")
     ap.add_argument("--block", type=int, default=0, help="threads per work-group (0 = default)")
     ap.add_argument("--generic", action="store_true", help="one-particle-per-wave kernels instead of pencils")
+    ap.add_argument("--no-mfma", dest="mfma", action="store_false",
+                    help="P = 16 FP32 gather on the vector register-ring kernel instead of the matrix cores")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has exactly one; "

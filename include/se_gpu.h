@@ -65,6 +65,9 @@ typedef struct {
 /* Use the generic one-particle-per-wave kernels even where the FP32 register-ring
  * (pencil) kernels exist (for A/B comparisons and tests). */
 #define SE_PLAN_GENERIC_KERNELS 1
+/* Keep the P = 16 FP32 gather on the vector register-ring kernel instead of the matrix
+ * cores (MFMA); for A/B comparisons and tests. */
+#define SE_PLAN_NO_MFMA 2
 
 int se_plan_create(se_plan** plan, const se_plan_desc* desc, void* hip_stream);
 int se_plan_destroy(se_plan* plan);

@@ -65,6 +65,7 @@ class se_plan_desc(C.Structure):
                 ("slab", C.c_double), ("Mz", C.c_int)]
 
 PLAN_GENERIC_KERNELS = 1
+PLAN_NO_MFMA = 2
 
 
 class se_job_shape(C.Structure):
@@ -247,11 +248,12 @@ class Plan:
     solve_group."""
 
     def __init__(self, M, P, box, xi, n_max, precision=PREC_FP64, m=0.0, tile=0, chunk=0, block=0,
-                 stream=None, generic=False, periodicity=3, slab=0.0, Mz=0, max_systems=1):
+                 stream=None, generic=False, periodicity=3, slab=0.0, Mz=0, max_systems=1, mfma=True):
         import torch
         self._torch = torch
         d = se_plan_desc(periodicity, M, P, float(box), float(xi), float(m), precision, int(n_max), tile, chunk,
-                         block, PLAN_GENERIC_KERNELS if generic else 0, float(slab), int(Mz))
+                         block, (PLAN_GENERIC_KERNELS if generic else 0) | (0 if mfma else PLAN_NO_MFMA),
+                         float(slab), int(Mz))
         self.desc = d
         self.max_systems = int(max_systems)
         self.stream = stream if stream is not None else torch.cuda.current_stream()

@@ -53,18 +53,19 @@ struct DevMem {
 };
 
 // Tiling rule.  Spread sub-grids accumulate in FP64 LDS (S^3 * 8 bytes), gather
-// sub-grids hold the grid precision.  Default tile side: the largest (<= 16; even
-// and <= 12 for the pencil kernels) whose FP64 sub-grid fits in LDS.
+// sub-grids hold the grid precision.  Default tile side: the largest (<= 16; a multiple
+// of 4 and <= 12 for the pencil kernels) whose FP64 sub-grid fits in LDS.
 static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sys, int tile, int chunk, int block,
-                            int lds_max, bool allow_pencil)
+                            int lds_max, bool allow_pencil, bool allow_mfma)
 {
     Tiling tl{};
     tl.pencil = (allow_pencil && esize == 4 && pencil_supported(P)) ? 1 : 0;
+    tl.mfma = (tl.pencil && allow_mfma && P == 16) ? 1 : 0;
     const int tmax = tl.pencil ? 12 : 16;
     int TS = tile;
     if (TS <= 0) {
         TS = 0;
-        for (int t = tmax; t >= 1; t -= (tl.pencil ? 2 : 1)) {
+        for (int t = tmax; t >= 1; t -= (tl.pencil ? 4 : 1)) {
             long S = t + P - 1;
             if (S * S * S * 8 <= lds_max) { TS = t; break; }
         }
@@ -72,9 +73,10 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     }
     const int Mmin = std::min(M, Mz);
     TS = std::min(TS, Mmin);
-    if (tl.pencil && (TS & 1)) TS -= 1;   // pencils pair columns: even tile side
-    if (tl.pencil && TS < 2) {
+    if (tl.pencil) TS -= TS % 4;   // pencils are 4 columns wide in x: tile side a multiple of 4
+    if (tl.pencil && TS < 4) {
         tl.pencil = 0;
+        tl.mfma = 0;
         TS = std::max(1, std::min(tile > 0 ? tile : Mmin, Mmin));
     }
     SE_REQUIRE(TS >= 1 && TS <= 255, SE_ERR_ARG, "tile side out of range");
@@ -85,8 +87,8 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     tl.Szp = tl.S;
     SE_REQUIRE((long)tl.S * tl.S * tl.Szp * 8 <= lds_max, SE_ERR_ARG, "tile sub-grid exceeds LDS");
     tl.ntiles = tl.NT * tl.NT * tl.NTz;
-    tl.PX = (TS + 1) / 2;
-    tl.K = tl.PX * tl.PX * TS;
+    tl.PX = (TS + 3) / 4;
+    tl.K = tl.PX * TS * TS;
     SE_REQUIRE((long)tl.ntiles * tl.K < (1l << 30), SE_ERR_ARG, "grid too large for the sort keys");
     tl.nbins = tl.ntiles * tl.K;
     tl.chunk = chunk > 0 ? chunk : 2048;
@@ -400,7 +402,7 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
     derive(d, g, eta, m, R2p);
     const int es = d->precision == SE_PREC_FP32 ? 4 : 8;
     Tiling tl = choose_tiling(es, d->P, g.M, g.Mz, d->n_max, n_sys, d->tile, d->chunk, d->block, lds_max,
-                              !(d->flags & SE_PLAN_GENERIC_KERNELS));
+                              !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA));
     const size_t M = g.M, Mz = g.Mz, N = d->n_max, ns = n_sys;
     size_t b = 0;
     b += 2 * N * 4;                                       // key, rank
@@ -653,7 +655,7 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         pl->n_cap = std::max(d->n_max, 1);
         const int lds_max = lds_limit();
         pl->tl = choose_tiling(pl->esize, d->P, pl->g.M, pl->g.Mz, pl->n_cap, n_sys, d->tile, d->chunk, d->block,
-                               lds_max, !(d->flags & SE_PLAN_GENERIC_KERNELS));
+                               lds_max, !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA));
         const Tiling& tl = pl->tl;
         const size_t M = pl->g.M, Mz = pl->g.Mz, N = pl->n_cap, ns = n_sys;
         SE_REQUIRE(ns * tl.nbins < (1ul << 31) - 1 && ns * tl.ntiles * (size_t)tl.K < (1ul << 31) - 1, SE_ERR_ARG,

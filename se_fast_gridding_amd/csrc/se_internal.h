@@ -65,8 +65,8 @@ struct Tiling {
     int NTz;          // tiles in z (ceil(Mz/TS))
     int Szp;          // z pitch of the LDS sub-grid (>= S)
     int ntiles;       // NT * NT * NTz
-    int PX;           // 2x2-column pencils per tile side: ceil(TS/2)
-    int K;            // sort keys per tile: PX*PX*TS
+    int PX;           // pencils (4 columns wide in x, 1 in z) per tile side in x: ceil(TS/4)
+    int K;            // sort keys per tile: PX*TS*TS
     int nbins;        // ntiles*K (the discard bucket is nbins)
     int chunk;        // max particles per work item
     int block;        // threads per work-group
@@ -75,6 +75,7 @@ struct Tiling {
     int max_items;    // ntiles + n_cap / chunk + 1
     int nseg;         // max merge segments per tile coordinate (segment-table stride)
     int pencil;       // 1: register-ring pencil kernels (FP32, P in the compiled set)
+    int mfma;         // 1: the pencil gather runs on MFMA (P = 16)
 };
 
 // Particle record written by the bucketing scatter, read by spread and gather.

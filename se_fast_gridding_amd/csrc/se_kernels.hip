@@ -119,8 +119,9 @@ __device__ __forceinline__ int window_start_z(double z, const Geom& g, double& t
 }
 
 // Counting-sort key of a particle: its window start cell, ordered tile-major, then
-// by 2x2-column pencil (px, pz), then by row y inside the tile:
-//   key = tile * K + ((px * PX + pz) * TS + ly),  K = PX * PX * TS,  PX = ceil(TS/2).
+// by pencil -- 4 window-start columns in x (px = lx / 4) by 1 in z (lz) --, then by
+// row y inside the tile:
+//   key = tile * K + ((px * TS + lz) * TS + ly),  K = PX * TS * TS,  PX = ceil(TS/4).
 // Sorting by this key groups particles by tile (for the work items) and, inside a
 // tile, into pencils walked row by row (for the pencil kernels).
 __device__ __forceinline__ int row_key(int w0, int w1, int w2, const Tiling& tl)
@@ -128,7 +129,7 @@ __device__ __forceinline__ int row_key(int w0, int w1, int w2, const Tiling& tl)
     const int tx = w0 / tl.TS, ty = w1 / tl.TS, tz = w2 / tl.TS;
     const int lx = w0 - tx * tl.TS, ly = w1 - ty * tl.TS, lz = w2 - tz * tl.TS;
     const int tile = (tx * tl.NT + ty) * tl.NTz + tz;
-    return tile * tl.K + (((lx >> 1) * tl.PX + (lz >> 1)) * tl.TS + ly);
+    return tile * tl.K + (((lx >> 2) * tl.TS + lz) * tl.TS + ly);
 }
 
 template <typename T>

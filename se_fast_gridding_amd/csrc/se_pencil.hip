@@ -1,10 +1,10 @@
 // Register-ring "pencil" kernels: the FP32 fast path of the fused engine.
 //
-// Particles are sorted (tile, pencil, row): a pencil is a 2x2 group of window-start
-// columns (x, z) inside a tile, walked along y.  All particles of a pencil touch the
-// same (P+1) x (P+1) block of (x, z) grid columns, so a wave keeps that block in
-// registers: lane l owns the (i, k) pairs l, l+64, ... (R = ceil((P+1)^2/64) per
-// lane) for P consecutive grid rows -- a ring of P rows.  Slot s of the ring holds
+// Particles are sorted (tile, pencil, row): a pencil is a group of 4 x 1 window-start
+// columns (4 in x, 1 in z) inside a tile, walked along y.  All particles of a pencil
+// touch the same (P+3) x P block of (x, z) grid columns, so a wave keeps that block in
+// registers: lane l owns the (i, k) pairs l, l+64, ... (R = ceil((P+3)P/64) per lane)
+// for P consecutive grid rows -- a ring of P rows.  Slot s of the ring holds
 // the row rho == s (mod P); instead of rotating registers when the window moves,
 // the y-factor is rotated: slot s of a particle at row y uses Y[(s - y) mod P].
 // Every register index is therefore static and each grid update is one v_fmac_f32
@@ -113,25 +113,25 @@ __device__ __forceinline__ float wave_sum_dpp(float v)
     return rlane(v, 63);
 }
 
-// Lane-owned (i, k) pairs of the (P+1)^2 pencil block: offset i*S + k in an LDS row
+// Lane-owned (i, k) pairs of the (P+3) x P pencil block: offset i*S + k in an LDS row
 // plane, and the ds_bpermute byte addresses (4*i, 4*k) that fetch the pair's x and z
-// factors from the lanes that computed them.  Padding slots (pair index >= (P+1)^2,
+// factors from the lanes that computed them.  Padding slots (pair index >= (P+3)P,
 // only in the last register) are mapped onto pairs of the first register, so every
 // lane addresses a distinct, in-bounds location without a branch, and fetch their x
-// factor from lane 63, whose factor is always 0 (63 >= P + 1).
+// factor from lane 63, whose factor is always 0 (63 >= P + 3).
 template <int P, int R>
 __device__ __forceinline__ void lane_pairs(int lane, int S, int (&pi)[R], int (&pk)[R], int (&pofs)[R],
                                            int (&bx)[R], int (&bz)[R])
 {
-    constexpr int W = P + 1;
-    static_assert(W < 63, "lane 63 must lie outside the window");
+    constexpr int WX = P + 3, WZ = P;
+    static_assert(WX < 63, "lane 63 must lie outside the window");
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int pr0 = lane + 64 * r;
-        const bool ok = pr0 < W * W;
-        const int pr = ok ? pr0 : pr0 % (W * W);
-        pi[r] = pr / W;
-        pk[r] = pr - (pr / W) * W;
+        const bool ok = pr0 < WX * WZ;
+        const int pr = ok ? pr0 : pr0 % (WX * WZ);
+        pi[r] = pr / WZ;
+        pk[r] = pr - (pr / WZ) * WZ;
         pofs[r] = pi[r] * S + pk[r];
         bx[r] = 4 * (ok ? pi[r] : 63);
         bz[r] = 4 * pk[r];
@@ -206,8 +206,7 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
                                                        const int* __restrict__ n_items, Geom g, Tiling tl,
                                                        float* __restrict__ partials)
 {
-    constexpr int W = P + 1;
-    constexpr int R = (W * W + 63) / 64;
+    constexpr int R = ((P + 3) * P + 63) / 64;
     const int item = blockIdx.x;
     if (item >= *n_items) return;   // uniform across the work-group
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -262,11 +261,12 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
     const float kc = -c * 1.4426950408889634f;
     const float olane = (float)(g.first + lane) * h;   // grid offset of window index `lane`
     auto factors = [&](const Rec<float>& rc, float (&sx)[R], float (&sz)[R], float& fy) {
-        const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
-        const int dx = lx & 1, dz = lz & 1;
-        // block column `lane` is window index lane - d (d = the particle's column in the pencil)
-        const float ax = rc.t[0] - olane + (float)dx * h, az = rc.t[2] - olane + (float)dz * h;
-        const bool vx = lane >= dx && lane < P + dx, vz = lane >= dz && lane < P + dz;
+        const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255;
+        const int dx = lx & 3;
+        // block column `lane` is window index lane - dx in x (dx = the particle's column in
+        // the pencil) and window index lane in z
+        const float ax = rc.t[0] - olane + (float)dx * h, az = rc.t[2] - olane;
+        const bool vx = lane >= dx && lane < P + dx, vz = lane < P;
         // the exponentials run on every lane and the masks select afterwards, so no
         // exec-mask branch is formed around them
         const float fx = gauss(kc, ax) * (vx ? rc.q * dn : 0.f);
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         const float fy = fy_n;
 
         const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
-        const int npen = ((lx >> 1) << 12) | (lz >> 1);
+        const int npen = ((lx >> 2) << 12) | lz;
         // one flush site (rows [base, upto) of the old window), so the accumulator ring
         // is never copied between control-flow paths
         int upto = base;
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
         if (npen != pen) {
             pen = npen;
             base = ly;
-            pbase = ((lx >> 1) * 2) * S + (lz >> 1) * 2;
+            pbase = ((lx >> 2) * 4) * S + lz;
         } else if (ly > base) {
             base = ly;
         }
@@ -351,8 +351,7 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
                                                        double* __restrict__ phi, double* __restrict__ force,
                                                        const SysDesc* __restrict__ sys)
 {
-    constexpr int W = P + 1;
-    constexpr int R = (W * W + 63) / 64;
+    constexpr int R = ((P + 3) * P + 63) / 64;
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -419,10 +418,10 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
         cozr[r] = c * ((float)(g.first + pk[r]) * h);
     }
     auto factors = [&](const Rec<float>& rc, float (&sx)[R], float (&sz)[R], float& fy, float& fyd) {
-        const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
-        const int dx = lx & 1, dz = lz & 1;
-        const float ax = rc.t[0] - olane + (float)dx * h, az = rc.t[2] - olane + (float)dz * h;
-        const bool vx = lane >= dx && lane < P + dx, vz = lane >= dz && lane < P + dz;
+        const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255;
+        const int dx = lx & 3;
+        const float ax = rc.t[0] - olane + (float)dx * h, az = rc.t[2] - olane;
+        const bool vx = lane >= dx && lane < P + dx, vz = lane < P;
         const float fx = gauss(kc, ax) * (vx ? dn : 0.f);
         const float fz = gauss(kc, az) * (vz ? 1.f : 0.f);
         // every 16-lane row holds the slot-rotated y factors (lanes s < P of the row)
@@ -479,13 +478,13 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
         const float fy = fy_n, fyd = fyd_n;
 
         const int lx = rc.lo & 255, ly = (rc.lo >> 8) & 255, lz = rc.lo >> 16;
-        const int npen = ((lx >> 1) << 12) | (lz >> 1);
+        const int npen = ((lx >> 2) << 12) | lz;
         // one load site (rows [lo, hi) of the new window) for the same reason as the
         // spread's single flush site
         int lo = 0, hi = 0;
         if (npen != pen) {
             pen = npen;
-            pbase = ((lx >> 1) * 2) * S + (lz >> 1) * 2;
+            pbase = ((lx >> 2) * 4) * S + lz;
             lo = ly;
             hi = ly + P;
             base = ly;
@@ -507,7 +506,7 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
         // lane partials of particle p: sum xz A, and for the x/z force sum xz A (c o h - c t)
         // (the difference is formed per pair: subtracting c t sum xz A once per particle
         // instead cancels about one more bit)
-        const float ctx = c * (rc.t[0] + (float)(lx & 1) * h), ctz = c * (rc.t[2] + (float)(lz & 1) * h);
+        const float ctx = c * (rc.t[0] + (float)(lx & 3) * h), ctz = c * rc.t[2];
         float sp = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -527,6 +526,261 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
         pq = rc.q;
     }
     if (porig >= 0) finish(psp, ps0, ps1, ps2, porig, pq);
+}
+
+// ---------------------------------------------------------------------------------
+// MFMA gather for P = 16.  The y contraction of a pencil's (P+3) x P = 19 x 16 column
+// block runs on v_mfma_f32_4x4x1_16b_f32 (16 independent 4x4x1 outer products per
+// instruction; operand and result layouts: A and B one value per lane, lane 4b + i is
+// row/column i of block b; D register i, lane 4b + j holds D_b[i][j]).  Block b is z
+// column b of the block; A-row i of ring register r is x column i + 4r (r < 5: x < 20,
+// 19 used).  A group of up to 4 consecutive particles of one pencil whose window rows
+// all lie in a 20-row ring [y0, y0 + 20) is contracted in one pass over the 20 ring rows:
+// column j of B holds particle j's y factor (lane 4b + j), so
+//   D_r[i] (lane 4b + j) = sum_y G[y][x = i + 4r][z = b] * Y_j(y)
+// and a second set of accumulators takes Y_j(y) (t_y - o_y h) for the y force.  The
+// epilogue multiplies by the particle's x and z factors per lane and sums over the 16
+// blocks.  The ring is the pencil kernels' ring with 20 rows in A layout; particles come
+// in (tile, pencil, row) order, so y0 never decreases inside a pencil.
+//
+// The y and x factors are evaluated for every ring row and every x column of the block,
+// also the (at most 4) rows and (3) columns outside a particle's P-point window: those
+// lie at least 8.5 h from the particle, where the Gaussian is below e^{-c 72 h^2}
+// (1e-10 relative at the P = 16 shape), far below FP32 resolution, so no masks are
+// needed.  A factor depends only on the particle slot j and the row or column, so each is
+// evaluated once per group (on lane 4 s + j) and fetched across the 16 blocks with
+// ds_bpermute instead of being recomputed on every block.
+// ---------------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Sum over the 16 lanes 4b + j (b = 0..15) that share j = lane & 3: DPP row rotations by
+// 4 and 8 inside each 16-lane row, then row swaps between rows 0/1, 2/3 and between the
+// two halves (gfx950 v_permlane16_swap / v_permlane32_swap); every lane gets the total.
+__device__ __forceinline__ float sum_over_blocks(float v)
+{
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xF, 0xF, false));   // row_ror:4
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));   // row_ror:8
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+    const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+
+// Per-lane data of one group (this lane's particle: slot lane & 3 of the group).
+struct GroupLane {
+    float tyl;    // t_y - first h + ly h: the y offset of row rho is tyl - rho h
+    float txf;    // t_x - first h: the x offset of window column w is txf - w h
+    float tzf;    // t_z - first h
+    float oxf;    // the particle's column in the pencil (x window index = x column - oxf)
+    float q;
+    int orig;
+    bool act;     // the slot holds a particle of the group
+};
+
+template <bool POT, bool FORCE>
+__global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restrict__ rec,
+                                                       const int* __restrict__ item_tile,
+                                                       const int* __restrict__ item_beg,
+                                                       const int* __restrict__ item_end,
+                                                       const int* __restrict__ n_items,
+                                                       const float* __restrict__ grid, Geom g, Tiling tl, int N,
+                                                       double* __restrict__ phi, double* __restrict__ force,
+                                                       const SysDesc* __restrict__ sys)
+{
+    constexpr int P = 16, GY = 20, R = 5;
+    const int item = blockIdx.x;
+    if (item >= *n_items) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
+    const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    {
+        int tile = item_tile[item];
+        if (sys) {   // grouped solve: the item's system, its outputs and its grid
+            const int si = tile / tl.ntiles;
+            tile -= si * tl.ntiles;
+            grid += (long)si * M * M * Mz;
+            phi = sys[si].phi;
+            force = sys[si].force;
+            N = sys[si].N;
+        }
+        const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
+        for (int e = tid; e < S3; e += nthr) {
+            const int iz = e % S, r2 = e / S, ix = r2 % S, iy = r2 / S;
+            const int gx = (ox + ix) % M, gy = (oy + iy) % M, gz = (oz + iz) % Mz;
+            sub[e] = grid[((long)gx * M + gy) * Mz + gz];
+        }
+    }
+    __syncthreads();
+
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
+    const int blk = lane >> 2, q4 = lane & 3;   // block = z column; A-row = particle slot
+    const int beg = item_beg[item], end = item_end[item], n = end - beg;
+    const int wb = beg + (int)(((long)n * wave) / nw), we = beg + (int)(((long)n * (wave + 1)) / nw);
+    const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
+    const float kc = -c * 1.4426950408889634f;
+    const float fh = (float)g.first * h;
+    const double h3 = g.h * g.h * g.h;
+
+    // A element of ring register r in this lane: x column q4 + 4r, z column blk.  The
+    // column x = 19 (past the P + 3 = 19 columns of the block) reads column 0 instead: its
+    // x factor is below 1e-10 for every particle of the pencil, as is the y factor of
+    // ring rows past the sub-grid, which read its last row; every value stays finite.
+    int aofs[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int px = q4 + 4 * r;
+        aofs[r] = (px < P + 3 ? px : 0) * S + blk;
+    }
+
+    float G[GY][R];
+    int pen = -1, base = 0, pbase = 0;
+    // ring rows in [lo, hi) (all within [base, base + GY))
+    auto load = [&](int lo, int hi) {
+        if (hi <= lo) return;
+        const unsigned mask = ring_mask<GY>(lo, hi - lo);
+        const int bm = base % GY;
+#pragma unroll
+        for (int s = 0; s < GY; ++s) {
+            if (mask & (1u << s)) {
+                const int rho = min(base + (s >= bm ? s - bm : s - bm + GY), S - 1);
+                const float* rowp = sub + rho * S2 + pbase;
+#pragma unroll
+                for (int r = 0; r < R; ++r) G[s][r] = rowp[aofs[r]];
+            }
+        }
+    };
+
+    int b0 = wb;
+    LaneRec batch = load_lane_rec(rec, min(b0 + lane, max(we - 1, beg)));
+    // the next group from particle p0 (< we): up to 4 particles of p0's pencil whose rows lie
+    // in [y0, y0 + GY - P]; moves the ring and returns this lane's particle data
+    auto next_group = [&](int p0, int& k) -> GroupLane {
+        if (p0 + 4 > b0 + 64) {   // records p0 .. p0 + 3 must lie in the batch
+            b0 = p0;
+            batch = load_lane_rec(rec, min(b0 + lane, we - 1));
+        }
+        const int lo0 = __builtin_amdgcn_readlane(batch.b.x, p0 - b0);
+        const int lx0 = lo0 & 255, ly0 = (lo0 >> 8) & 255, lz0 = lo0 >> 16;
+        const int npen = ((lx0 >> 2) << 12) | lz0;
+        k = 1;
+#pragma unroll
+        for (int jj = 1; jj < 4; ++jj) {
+            if (k < jj || p0 + jj >= we) break;
+            const int loj = __builtin_amdgcn_readlane(batch.b.x, p0 - b0 + jj);
+            if (((((loj & 255) >> 2) << 12) | (loj >> 16)) != npen || ((loj >> 8) & 255) > ly0 + (GY - P)) break;
+            k = jj + 1;
+        }
+        int lo = 0, hi = 0;
+        if (npen != pen) {
+            pen = npen;
+            pbase = ((lx0 >> 2) * 4) * S + lz0;
+            base = ly0;
+            lo = ly0;
+            hi = ly0 + GY;
+        } else if (ly0 > base) {
+            lo = max(base + GY, ly0);
+            hi = ly0 + GY;
+            base = ly0;
+        }
+        load(lo, hi);
+        // slots past k repeat the group's last particle and store nothing
+        const int sa = 4 * (p0 - b0 + min(q4, k - 1));
+        const int loj = __builtin_amdgcn_ds_bpermute(sa, batch.b.x);
+        GroupLane d;
+        d.tyl = bperm(sa, batch.a.y) - fh + (float)((loj >> 8) & 255) * h;
+        d.txf = bperm(sa, batch.a.x) - fh;
+        d.tzf = bperm(sa, batch.a.z) - fh;
+        d.oxf = (float)(loj & 3);
+        d.q = bperm(sa, batch.a.w);
+        d.orig = __builtin_amdgcn_ds_bpermute(sa, batch.b.y);
+        d.act = q4 < k;
+        return d;
+    };
+
+    // y contraction of the group in `d` over the 20 ring rows into (Dp, Df).  A y factor
+    // depends only on (particle slot j, ring slot s), so lane l forms the factors of
+    // (j = l & 3, s = l >> 2) and (j, s = 16 + (l >> 2)) once, and each B operand column is
+    // fetched with a ds_bpermute (source lane 4 s + j); the MFMAs then issue back to back.
+    auto contract = [&](const GroupLane& d, v4f (&Dp)[R], v4f (&Df)[R]) {
+        const v4f zero4 = {0.f, 0.f, 0.f, 0.f};
+        const int bm = base % GY;
+        const int s1 = blk, s2 = min(16 + blk, GY - 1);
+        const int rho1 = base + (s1 >= bm ? s1 - bm : s1 - bm + GY);
+        const int rho2 = base + (s2 >= bm ? s2 - bm : s2 - bm + GY);
+        const float ay1 = fmaf(-(float)rho1, h, d.tyl), ay2 = fmaf(-(float)rho2, h, d.tyl);
+        const float y1 = __builtin_amdgcn_exp2f(kc * (ay1 * ay1)), y2 = __builtin_amdgcn_exp2f(kc * (ay2 * ay2));
+        const float b1 = y1 * ay1, b2 = y2 * ay2;
+        const int src = 4 * q4;
+        float yv[GY], yb[GY];
+#pragma unroll
+        for (int s = 0; s < GY; ++s) {
+            yv[s] = bperm(src + 16 * (s & 15), s < 16 ? y1 : y2);
+            if (FORCE) yb[s] = bperm(src + 16 * (s & 15), s < 16 ? b1 : b2);
+        }
+#pragma unroll
+        for (int s = 0; s < GY; ++s) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                Dp[r] = __builtin_amdgcn_mfma_f32_4x4x1f32(G[s][r], yv[s], s == 0 ? zero4 : Dp[r], 0, 0, 0);
+                if (FORCE)
+                    Df[r] = __builtin_amdgcn_mfma_f32_4x4x1f32(G[s][r], yb[s], s == 0 ? zero4 : Df[r], 0, 0, 0);
+            }
+        }
+    };
+
+    // x and z factors of this lane's columns, sums over the blocks, outputs of the group in d
+    auto epilogue = [&](const GroupLane& d, const v4f (&Dp)[R], const v4f (&Df)[R]) {
+        // x factors of (j = l & 3, column l >> 2) and (j, column 16 + (l >> 2)) on lane l,
+        // fetched per column like the y factors
+        const float ax1 = fmaf(d.oxf - (float)blk, h, d.txf), ax2 = fmaf(d.oxf - (float)(16 + blk), h, d.txf);
+        const float x1 = __builtin_amdgcn_exp2f(kc * (ax1 * ax1)), x2 = __builtin_amdgcn_exp2f(kc * (ax2 * ax2));
+        const float xa1 = x1 * ax1, xa2 = x2 * ax2;
+        const int src = 4 * q4;
+        float xv[4 * R], xa[4 * R];
+#pragma unroll
+        for (int c4 = 0; c4 < 4 * R; ++c4) {
+            xv[c4] = bperm(src + 16 * (c4 & 15), c4 < 16 ? x1 : x2);
+            if (FORCE) xa[c4] = bperm(src + 16 * (c4 & 15), c4 < 16 ? xa1 : xa2);
+        }
+        float S0 = 0.f, S1 = 0.f, S2 = 0.f;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                S0 = fmaf(xv[i + 4 * r], Dp[r][i], S0);
+                if (FORCE) {
+                    S1 = fmaf(xa[i + 4 * r], Dp[r][i], S1);
+                    S2 = fmaf(xv[i + 4 * r], Df[r][i], S2);
+                }
+            }
+        const float az = fmaf(-(float)blk, h, d.tzf);
+        const float zf = dn * __builtin_amdgcn_exp2f(kc * (az * az));
+        float sp = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
+        if (POT) sp = sum_over_blocks(zf * S0);
+        if (FORCE) {
+            s0 = sum_over_blocks(zf * S1);
+            s1 = sum_over_blocks(zf * S2);
+            s2 = sum_over_blocks((zf * az) * S0);
+        }
+        if (d.act && blk == 0) {
+            if (POT) store_out(phi, d.orig, h3 * (double)sp);
+            if (FORCE) {
+                // derivative factors -c (t - o h) of x, y and z
+                const double qh = -(double)c * h3 * (double)d.q;
+                store_out(force, d.orig, qh * (double)s0);
+                store_out(force, d.orig + (long)N, qh * (double)s1);
+                store_out(force, d.orig + 2l * N, qh * (double)s2);
+            }
+        }
+    };
+
+    v4f Dp[R], Df[R];
+    for (int p0 = wb, k = 0; p0 < we; p0 += k) {
+        const GroupLane d = next_group(p0, k);
+        contract(d, Dp, Df);
+        epilogue(d, Dp, Df);
+    }
 }
 
 bool pencil_supported(int P) { return P == 8 || P == 11 || P == 16; }
@@ -554,6 +808,28 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int*
     default: return false;
     }
 #undef SE_SP
+}
+
+static void interp_mfma16_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg,
+                                const int* item_end, const int* n_items, const float* grid, const Geom& g,
+                                const Tiling& tl, int N, double* phi, double* force, const SysDesc* sys,
+                                hipStream_t s)
+{
+    const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
+    dim3 gr(tl.max_items), bl(tl.block);
+#define SE_GM(POT, FORCE)                                                                                     \
+    do {                                                                                                      \
+        allow_lds_p((const void*)k_interp_mfma16<POT, FORCE>, lds);                                           \
+        hipLaunchKernelGGL((k_interp_mfma16<POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg, item_end, \
+                           n_items, grid, g, tl, N, phi, force, sys);                                         \
+    } while (0)
+    if (phi && force)
+        SE_GM(true, true);
+    else if (force)
+        SE_GM(false, true);
+    else
+        SE_GM(true, false);
+#undef SE_GM
 }
 
 template <int PV>
@@ -587,7 +863,12 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
     switch (g.P) {
     case 8: SE_GPM(8); return true;
     case 11: SE_GPM(11); return true;
-    case 16: SE_GPM(16); return true;
+    case 16:
+        if (tl.mfma)
+            interp_mfma16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys, s);
+        else
+            SE_GPM(16);
+        return true;
     default: return false;
     }
 #undef SE_GPM

@@ -197,6 +197,40 @@ def test_pencil_kernels_match_oracle_and_generic(oracle, P):
     assert rms_rel(p1.cpu().numpy(), ref_phi) < FP32_TOL and rms_rel(f1.cpu().numpy(), ref_F) < FP32_TOL
 
 
+@pytest.mark.parametrize("clustered", [False, True])
+def test_mfma_gather_matches_vector_gather(oracle, clustered):
+    """The P = 16 matrix-core gather (4-particle groups on a 20-row ring) against the
+    vector register-ring gather on the same spread grid, for each output mode, with work
+    items and waves that cut pencils and groups mid-way; and against the FP64 oracle."""
+    N, M, P, xi = 6000, 32, 16, 8.0
+    x, q = system(N, 300 + int(clustered), clustered=clustered)
+    m = 0.95 * np.sqrt(np.pi * P)
+    ref_phi = oracle.se3p(x, q, 1.0, M, P, m, xi)
+    ref_F = oracle.se3p(x, q, 1.0, M, P, m, xi, force=True)
+    xt, qt = torch.tensor(x, device=DEV), torch.tensor(q, device=DEV)
+    for kw in (dict(), dict(chunk=45, block=128), dict(tile=4, chunk=301, block=64)):
+        out = {}
+        for mf in (True, False):
+            plan = se.Plan(M, P, 1.0, xi, N, precision=se.PREC_FP32, mfma=mf, **kw)
+            phi = torch.zeros(N, dtype=torch.float64, device=DEV)
+            F = torch.zeros(3 * N, dtype=torch.float64, device=DEV)
+            p1 = torch.zeros(N, dtype=torch.float64, device=DEV)
+            f1 = torch.zeros(3 * N, dtype=torch.float64, device=DEV)
+            plan.solve(xt, qt, phi, F)
+            plan.interp(p1, None)
+            plan.interp(None, f1)
+            torch.cuda.synchronize()
+            assert plan.info()["pencil"] == 1
+            plan.close()
+            out[mf] = [t.cpu().numpy() for t in (phi, F, p1, f1)]
+        a, b = out[True], out[False]
+        print("mfma vs vector", kw, [rms_rel(u, v) for u, v in zip(a, b)], flush=True)
+        for u, v in zip(a, b):
+            assert rms_rel(u, v) < 2e-6, kw
+        assert rms_rel(a[2], a[0]) < 1e-7 and rms_rel(a[3], a[1]) < 1e-7, kw
+        assert rms_rel(a[0], ref_phi) < FP32_TOL and rms_rel(a[1], ref_F) < FP32_TOL, kw
+
+
 def clustered_system(N, seed=3, n_clusters=64, frac=0.8, sigma=0.01):
     """80 % of N in 64 Gaussian clusters (sigma = 0.01 L), 20 % uniform, q = +-1, zero net charge."""
     rng = np.random.default_rng(seed)
