@@ -96,6 +96,29 @@ def test_spread_grid_matches_oracle_wrap(oracle):
         assert np.max(np.abs(G - ref)) <= tol * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("clustered", [False, True])
+def test_pencil_spread_grid_matches_oracle_wrap(oracle, clustered):
+    """The P = 16 FP32 register-ring spread gives the reference's wrapped grid, also with
+    work items and waves that cut pencils mid-way (heavy clustered tiles split into
+    chunks, small work-groups, small tiles)."""
+    N, M, P, xi = 5000, 24, 16, 5.0
+    x, q = system(N, 40 + int(clustered), clustered=clustered)
+    p = oracle.pack_params(N, M, P, plan_c(M, P, xi), 1.0 / M)
+    zs = oracle.base_gaussian(p)
+    ref = oracle.wrap(3, p, oracle.grid_split(p, q, oracle.expand_all(3, x, p), zs)).reshape(M, M, M, order="F")
+    xt, qt = torch.tensor(x, device=DEV), torch.tensor(q, device=DEV)
+    for kw in (dict(), dict(chunk=57, block=128), dict(tile=4, chunk=300, block=64)):
+        plan = se.Plan(M, P, 1.0, xi, N, precision=se.PREC_FP32, **kw)
+        plan.spread(xt, qt)
+        torch.cuda.synchronize()
+        assert plan.info()["pencil"] == 1
+        G = plan.grid().double().cpu().numpy()
+        plan.close()
+        e = np.max(np.abs(G - ref)) / np.max(np.abs(ref))
+        print("pencil spread %s: max rel %.2e" % (kw, e), flush=True)
+        assert e < 2e-6, kw
+
+
 def plan_c(M, P, xi, L=1.0):
     m = 0.95 * np.sqrt(np.pi * P)
     h = L / M
