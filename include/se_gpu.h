@@ -168,6 +168,37 @@ int se_energy(se_plan* plan, int N, const double* d_phi, const double* d_q, doub
 const char* se_gpu_last_error(void);
 
 /*
+ * Real-space and self terms of the Ewald sum, so that real + k-space (se_solve) + self
+ * gives the full Coulomb sum of the periodic system.  Replaces the reference's direct
+ * evaluations SE3P_direct_real_rc (SE_direct.c:464-505), SE2P_direct_real_rc (:224-254),
+ * SE1P_direct_real_rc (:50-88) and SE*P_direct_self (:178-183, 340-345, 550-561), which
+ * loop over all pairs and image layers; here particles are bucketed into cells no smaller
+ * than the cutoff and each particle visits the neighbouring cells.
+ *   phi_m   = sum_{n, images, r <= rc, not the particle itself} q_n erfc(xi r)/r
+ *             (- 2 xi/sqrt(pi) q_m when `self` is set)
+ *   force_m = (q_m/2) grad phi_m, the convention of the reference's k-space force
+ *             (SE_direct.c:417-461), so it adds to se_solve's force.
+ * Periodic directions fold positions into [0, box); in free directions particles must
+ * lie in [0, box) and others are left out (their outputs are not written).  Outputs are
+ * assigned, or added into phi / force when `accumulate` is set.  Asynchronous on the
+ * plan's stream; arrays are device pointers in the reference layout.
+ */
+typedef struct {
+    int periodicity;   /* 3, 2 (x and y periodic) or 1 (z periodic) */
+    double box[3];     /* period of each periodic direction; extent of the particles' range in a free one */
+    double xi;         /* Ewald parameter */
+    double rc;         /* cutoff radius */
+    int self;          /* 1: include the self term in phi */
+    int accumulate;    /* 1: add into phi / force instead of assigning */
+} se_real_desc;
+
+typedef struct se_real_plan se_real_plan;
+int se_real_plan_create(se_real_plan** plan, const se_real_desc* desc, int n_max, void* hip_stream);
+int se_real_plan_destroy(se_real_plan* plan);
+/* phi and/or force may be NULL (not computed). */
+int se_real_solve(se_real_plan* plan, int N, const double* d_x, const double* d_q, double* d_phi, double* d_force);
+
+/*
  * Batched and served solves (no reference counterpart: the reference solves one system
  * per call).  An executor owns `n_slots` HIP streams and a per-slot cache of plans
  * keyed by geometry; jobs are independent systems in device memory.  se_batch_submit

@@ -86,6 +86,12 @@ class se_system(C.Structure):
                 ("d_force", C.c_void_p)]
 
 
+class se_real_desc(C.Structure):
+    """se_gpu.h se_real_desc."""
+    _fields_ = [("periodicity", C.c_int), ("box", C.c_double * 3), ("xi", C.c_double), ("rc", C.c_double),
+                ("self", C.c_int), ("accumulate", C.c_int)]
+
+
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int)
 _PP = C.POINTER(SE_FGG_params)
@@ -173,6 +179,9 @@ _SIG = {
     "se_batch_wait": (C.c_int, [_VP, C.c_int]),
     "se_batch_wait_all": (C.c_int, [_VP]),
     "se_batch_stats": (C.c_int, [_VP, C.POINTER(C.c_size_t), _I]),
+    "se_real_plan_create": (C.c_int, [C.POINTER(_VP), C.POINTER(se_real_desc), C.c_int, _VP]),
+    "se_real_plan_destroy": (C.c_int, [_VP]),
+    "se_real_solve": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP]),
 }
 
 _SPREAD_SPLIT = ["SE_FGG_grid_split_SSE_dispatch", "SE_FGG_grid_split", "SE_FGG_grid_split_SSE",
@@ -348,6 +357,32 @@ class Plan:
 
     def nbytes(self):
         return int(capi().se_plan_bytes(self.h))
+
+
+class RealSpace:
+    """Real-space (+ self) Ewald terms on the device (se_gpu.h se_real_*): phi_m = sum over
+    pairs and images within rc of q_n erfc(xi r)/r, force = (q/2) grad phi; periodicity 3,
+    2 (x, y periodic) or 1 (z periodic); box = periods / extents of the free directions."""
+
+    def __init__(self, periodicity, box, xi, rc, n_max, self_term=False, accumulate=False, stream=None):
+        import torch
+        box = [float(b) for b in (box if hasattr(box, "__len__") else (box, box, box))]
+        d = se_real_desc(periodicity, (C.c_double * 3)(*box), float(xi), float(rc), int(self_term), int(accumulate))
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        h = C.c_void_p()
+        check(capi().se_real_plan_create(C.byref(h), C.byref(d), int(n_max), C.c_void_p(self.stream.cuda_stream)),
+              "se_real_plan_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            capi().se_real_plan_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def solve(self, x, q, phi=None, force=None):
+        check(capi().se_real_solve(self.h, q.numel(), _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_real_solve")
 
 
 _hip = None
