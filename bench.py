@@ -318,9 +318,29 @@ def run_batched(args):
         b.close()
         return dt, phis, st
 
+    tuned = {}
+    t32u = None
+    if args.tune:
+        # untuned grouped run first, then the autotuner (se_tune) per geometry on a group of
+        # that geometry's size (its system count at the mean system size); the group plans
+        # created afterwards take its tilings
+        se.capi().se_tune_clear()
+        t32u, _, _ = timed(se.PREC_FP32, args.slots, True)
+        shapes = {}
+        for s in systems:
+            shapes.setdefault((s["per"], s["M"], s["P"], s["xi"]), []).append(s["N"])
+        t0 = time.perf_counter()
+        for (per, M, P, xi), ns in sorted(shapes.items()):
+            t, c, b, ms = se.tune(M, P, 1.0, xi, int(np.mean(ns)), precision=se.PREC_FP32, periodicity=per,
+                                  slab=1.0 if per == 2 else 0.0, potential=True, force=False, systems=len(ns))
+            tuned["SE%dP M=%d P=%d" % (per, M, P)] = {"tile": t, "chunk": c, "block": b, "group_ms": round(ms, 4),
+                                                     "N": int(np.mean(ns)), "systems": len(ns)}
+        tune_s = time.perf_counter() - t0
     t32, phi32, st32 = timed(se.PREC_FP32, args.slots, True)
     t32s, phi32s, _ = timed(se.PREC_FP32, args.slots, False)
     t64, phi64, _ = timed(se.PREC_FP64, 1, False)
+    if args.tune:
+        se.capi().se_tune_clear()
 
     def errs_of(p):
         return np.array([float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)) for a, b in zip(p, phi64)])
@@ -341,6 +361,9 @@ def run_batched(args):
                           "phi_rms_rel_max": float(errs_s.max())},
         "plan_cache": st32,
     }
+    if args.tune:
+        out["autotune"] = {"tilings": tuned, "tune_s": round(tune_s, 2),
+                           "untuned_solves_per_s": n_sys / t32u, "tuned_over_untuned": t32u / t32}
     print(json.dumps(out))
 
 
@@ -441,6 +464,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="max particles per work item (0 = default)")
     ap.add_argument("--block", type=int, default=0, help="threads per work-group (0 = default)")
     ap.add_argument("--generic", action="store_true", help="one-particle-per-wave kernels instead of pencils")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="batched: skip the tiling autotuner (se_tune) before the timed runs")
     ap.add_argument("--no-mfma", dest="mfma", action="store_false",
                     help="P = 16 FP32 gather on the vector register-ring kernel instead of the matrix cores")
     args = ap.parse_args()

@@ -168,6 +168,27 @@ int se_energy(se_plan* plan, int N, const double* d_phi, const double* d_q, doub
 const char* se_gpu_last_error(void);
 
 /*
+ * Tiling autotuner (no reference counterpart).  se_tune times the full solve (the outputs
+ * in `quantities`: SE_Q_POTENTIAL and/or SE_Q_FORCE) of `systems` synthetic uniform
+ * systems of N particles each on desc's geometry -- one plan for systems = 1, a grouped
+ * solve (se_solve_group) otherwise -- under every candidate tiling -- tile side, maximum
+ * particles per work item (the bucket cap that splits heavy tiles) and threads per
+ * work-group -- and records the fastest in a process-wide table keyed by (periodicity,
+ * M, P, precision, ceil(log2 N)).  Plans created afterwards with tile = chunk = block = 0
+ * take the recorded tiling of their geometry at the nearest size band (group plans use
+ * particles per system), including the plans the batch executor creates.  out = {tile, chunk, block} and *ms =
+ * its solve time.  se_tune allocates its own scratch and synchronises `hip_stream`.
+ * se_tune_save / se_tune_load write and read the table as text (load returns the number
+ * of entries read, or an error code); se_tune_lookup returns 1 and the tiling a plan of
+ * desc's geometry with N particles would take, else 0.
+ */
+int se_tune(const se_plan_desc* desc, int N, int systems, int quantities, void* hip_stream, int out[3], float* ms);
+int se_tune_save(const char* path);
+int se_tune_load(const char* path);
+int se_tune_clear(void);
+int se_tune_lookup(const se_plan_desc* desc, int N, int out[3]);
+
+/*
  * Real-space and self terms of the Ewald sum, so that real + k-space (se_solve) + self
  * gives the full Coulomb sum of the periodic system.  Replaces the reference's direct
  * evaluations SE3P_direct_real_rc (SE_direct.c:464-505), SE2P_direct_real_rc (:224-254),

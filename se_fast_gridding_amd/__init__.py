@@ -182,6 +182,11 @@ _SIG = {
     "se_real_plan_create": (C.c_int, [C.POINTER(_VP), C.POINTER(se_real_desc), C.c_int, _VP]),
     "se_real_plan_destroy": (C.c_int, [_VP]),
     "se_real_solve": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP]),
+    "se_tune": (C.c_int, [C.POINTER(se_plan_desc), C.c_int, C.c_int, C.c_int, _VP, _I, C.POINTER(C.c_float)]),
+    "se_tune_save": (C.c_int, [C.c_char_p]),
+    "se_tune_load": (C.c_int, [C.c_char_p]),
+    "se_tune_clear": (C.c_int, []),
+    "se_tune_lookup": (C.c_int, [C.POINTER(se_plan_desc), C.c_int, _I]),
 }
 
 _SPREAD_SPLIT = ["SE_FGG_grid_split_SSE_dispatch", "SE_FGG_grid_split", "SE_FGG_grid_split_SSE",
@@ -357,6 +362,30 @@ class Plan:
 
     def nbytes(self):
         return int(capi().se_plan_bytes(self.h))
+
+
+def tune(M, P, box, xi, N, precision=PREC_FP32, periodicity=3, slab=0.0, m=0.0, potential=True, force=True,
+         systems=1, stream=None):
+    """se_tune: time every candidate tiling on `systems` synthetic uniform systems of N
+    particles (a grouped solve when systems > 1) and record the fastest for plans of this
+    geometry and size band; returns (tile, chunk, block, ms)."""
+    import torch
+    d = se_plan_desc(periodicity, M, P, float(box), float(xi), float(m), precision, int(N), 0, 0, 0, 0,
+                     float(slab), 0)
+    out = (C.c_int * 3)()
+    ms = C.c_float()
+    st = stream if stream is not None else torch.cuda.current_stream()
+    qty = (1 if potential else 0) | (2 if force else 0)
+    check(capi().se_tune(C.byref(d), int(N), int(systems), qty, C.c_void_p(st.cuda_stream), out, C.byref(ms)),
+          "se_tune")
+    return out[0], out[1], out[2], ms.value
+
+
+def tune_lookup(M, P, box, xi, N, precision=PREC_FP32, periodicity=3, slab=0.0):
+    """The recorded tiling (tile, chunk, block) for this geometry and size, or None."""
+    d = se_plan_desc(periodicity, M, P, float(box), float(xi), 0.0, precision, int(N), 0, 0, 0, 0, float(slab), 0)
+    out = (C.c_int * 3)()
+    return tuple(out) if capi().se_tune_lookup(C.byref(d), int(N), out) == 1 else None
 
 
 class RealSpace:

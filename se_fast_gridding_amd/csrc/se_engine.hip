@@ -401,7 +401,9 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
     double R2p = 0;
     derive(d, g, eta, m, R2p);
     const int es = d->precision == SE_PREC_FP32 ? 4 : 8;
-    Tiling tl = choose_tiling(es, d->P, g.M, g.Mz, d->n_max, n_sys, d->tile, d->chunk, d->block, lds_max,
+    int tile = d->tile, chunk = d->chunk, block = d->block;
+    if (tile == 0 && chunk == 0 && block == 0) tuned_tiling(d, std::max(d->n_max, 1) / n_sys, tile, chunk, block);
+    Tiling tl = choose_tiling(es, d->P, g.M, g.Mz, d->n_max, n_sys, tile, chunk, block, lds_max,
                               !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA));
     const size_t M = g.M, Mz = g.Mz, N = d->n_max, ns = n_sys;
     size_t b = 0;
@@ -654,8 +656,10 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         pl->stream = (hipStream_t)stream;
         pl->n_cap = std::max(d->n_max, 1);
         const int lds_max = lds_limit();
-        pl->tl = choose_tiling(pl->esize, d->P, pl->g.M, pl->g.Mz, pl->n_cap, n_sys, d->tile, d->chunk, d->block,
-                               lds_max, !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA));
+        int tile = d->tile, chunk = d->chunk, block = d->block;
+        if (tile == 0 && chunk == 0 && block == 0) tuned_tiling(d, pl->n_cap / n_sys, tile, chunk, block);
+        pl->tl = choose_tiling(pl->esize, d->P, pl->g.M, pl->g.Mz, pl->n_cap, n_sys, tile, chunk, block, lds_max,
+                               !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA));
         const Tiling& tl = pl->tl;
         const size_t M = pl->g.M, Mz = pl->g.Mz, N = pl->n_cap, ns = n_sys;
         SE_REQUIRE(ns * tl.nbins < (1ul << 31) - 1 && ns * tl.ntiles * (size_t)tl.K < (1ul << 31) - 1, SE_ERR_ARG,

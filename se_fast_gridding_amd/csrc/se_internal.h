@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <string>
 
+#include "../../include/se_gpu.h"
+
 namespace se {
 
 // ---------------------------------------------------------------------------------
@@ -113,6 +115,10 @@ struct LParams {
     double c, d, h, a, b;
     int per;
 };
+
+// ---- tuned tilings (se_tune.hip) ---------------------------------------------------
+// The tiling se_tune recorded for d's geometry and n particles per system, if any.
+bool tuned_tiling(const se_plan_desc* d, long n_per_system, int& tile, int& chunk, int& block);
 
 // ---- launchers (se_kernels.hip) ------------------------------------------------
 void launch_fill_i32(int* p, long n, int v, hipStream_t s);

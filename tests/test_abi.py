@@ -139,3 +139,28 @@ int main(void) {
     got = _compile_run(prog, se.INCLUDE_DIR)
     D = se.se_plan_desc
     assert [int(v) for v in got] == [C.sizeof(D), D.box.offset, D.flags.offset, D.slab.offset, D.Mz.offset]
+
+
+def test_tuning_table_save_load_lookup(tmp_path):
+    """The autotuner's table: a saved table loads back, lookups take the geometry's entry at
+    the nearest size band, other geometries have none, and clear empties it (host only)."""
+    lib = se.capi()
+    lib.se_tune_clear()
+    path = tmp_path / "tune.txt"
+    path.write_text("# periodicity M P fp32 log2N tile chunk block ms\n"
+                    "3 64 16 1 14 8 1024 256 0.5\n"
+                    "3 64 16 1 20 12 4096 512 2.0\n"
+                    "2 48 11 1 15 4 512 512 0.3\n")
+    assert lib.se_tune_load(str(path).encode()) == 3
+    assert se.tune_lookup(64, 16, 1.0, 20.0, 16000) == (8, 1024, 256)       # band 14
+    assert se.tune_lookup(64, 16, 1.0, 20.0, 900000) == (12, 4096, 512)     # band 20
+    assert se.tune_lookup(64, 16, 1.0, 20.0, 60000) == (8, 1024, 256)       # band 16: nearer 14
+    assert se.tune_lookup(48, 11, 1.0, 15.0, 30000, periodicity=2, slab=1.0) == (4, 512, 512)
+    assert se.tune_lookup(64, 16, 1.0, 20.0, 16000, precision=se.PREC_FP64) is None
+    out = tmp_path / "out.txt"
+    assert lib.se_tune_save(str(out).encode()) == 0
+    lib.se_tune_clear()
+    assert se.tune_lookup(64, 16, 1.0, 20.0, 16000) is None
+    assert lib.se_tune_load(str(out).encode()) == 3
+    assert se.tune_lookup(64, 16, 1.0, 20.0, 16000) == (8, 1024, 256)
+    lib.se_tune_clear()

@@ -133,3 +133,35 @@ def test_graph_replays_match_eager_solves(spec):
     assert rms_rel(phi_g.cpu().numpy(), phi_e.cpu().numpy()) <= tol
     assert rms_rel(F_g.cpu().numpy(), F_e.cpu().numpy()) <= tol
     plan.close()
+
+
+def test_tune_records_a_tiling_that_new_plans_use():
+    """se_tune times the candidate tilings and records the fastest; a plan created
+    afterwards with the automatic tiling takes it and solves the same system to FP32
+    rounding of the default plan."""
+    lib = se.capi()
+    lib.se_tune_clear()
+    try:
+        M, P, xi, N = 32, 16, 10.0, 20000
+        t, c, b, ms = se.tune(M, P, 1.0, xi, N)
+        assert t in (4, 8, 12) and c in (512, 1024, 2048, 4096) and b in (256, 512) and ms > 0
+        assert se.tune_lookup(M, P, 1.0, xi, N) == (t, c, b)
+        rng = np.random.default_rng(3)
+        x = torch.tensor(rng.random(3 * N), device="cuda")
+        q = torch.tensor(rng.uniform(-1, 1, N), device="cuda")
+        out = []
+        for tuned in (True, False):
+            if not tuned:
+                lib.se_tune_clear()
+            plan = se.Plan(M, P, 1.0, xi, N, precision=se.PREC_FP32)
+            info = plan.info()
+            if tuned:
+                assert (info["tile"], info["chunk"], info["block"]) == (t, c, b)
+            phi = torch.zeros(N, dtype=torch.float64, device="cuda")
+            plan.solve(x, q, phi, None)
+            torch.cuda.synchronize()
+            plan.close()
+            out.append(phi.cpu().numpy())
+        assert np.linalg.norm(out[0] - out[1]) / np.linalg.norm(out[1]) < 2e-6
+    finally:
+        lib.se_tune_clear()
