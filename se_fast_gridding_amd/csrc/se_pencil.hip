@@ -22,6 +22,8 @@
 // whenever its range crosses into another pencil.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "se_internal.h"
 
 namespace se {
@@ -349,11 +351,13 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
                                                        const int* __restrict__ n_items,
                                                        const float* __restrict__ grid, Geom g, Tiling tl, int N,
                                                        double* __restrict__ phi, double* __restrict__ force,
-                                                       const SysDesc* __restrict__ sys)
+                                                       const SysDesc* __restrict__ sys, int dense_min)
 {
     constexpr int R = ((P + 3) * P + 63) / 64;
     const int item = blockIdx.x;
     if (item >= *n_items) return;
+    // items of dense_min particles or more are the MFMA gather's (dense_min > 0)
+    if (dense_min > 0 && item_end[item] - item_beg[item] >= dense_min) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -585,11 +589,13 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
                                                        const int* __restrict__ n_items,
                                                        const float* __restrict__ grid, Geom g, Tiling tl, int N,
                                                        double* __restrict__ phi, double* __restrict__ force,
-                                                       const SysDesc* __restrict__ sys)
+                                                       const SysDesc* __restrict__ sys, int dense_min)
 {
     constexpr int P = 16, GY = 20, R = 5;
     const int item = blockIdx.x;
     if (item >= *n_items) return;
+    // items below dense_min particles are left to the vector gather
+    if (item_end[item] - item_beg[item] < dense_min) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -813,7 +819,7 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int*
 static void interp_mfma16_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg,
                                 const int* item_end, const int* n_items, const float* grid, const Geom& g,
                                 const Tiling& tl, int N, double* phi, double* force, const SysDesc* sys,
-                                hipStream_t s)
+                                int dense_min, hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
     dim3 gr(tl.max_items), bl(tl.block);
@@ -821,7 +827,7 @@ static void interp_mfma16_modes(const Rec<float>* rec, const int* item_tile, con
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_mfma16<POT, FORCE>, lds);                                           \
         hipLaunchKernelGGL((k_interp_mfma16<POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg, item_end, \
-                           n_items, grid, g, tl, N, phi, force, sys);                                         \
+                           n_items, grid, g, tl, N, phi, force, sys, dense_min);                              \
     } while (0)
     if (phi && force)
         SE_GM(true, true);
@@ -836,7 +842,7 @@ template <int PV>
 static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg,
                                 const int* item_end, const int* n_items, const float* grid, const Geom& g,
                                 const Tiling& tl, int N, double* phi, double* force, const SysDesc* sys,
-                                hipStream_t s)
+                                int dense_min, hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
     dim3 gr(tl.max_items), bl(tl.block);
@@ -844,7 +850,7 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_pencil<PV, POT, FORCE>, lds);                                       \
         hipLaunchKernelGGL((k_interp_pencil<PV, POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg,       \
-                           item_end, n_items, grid, g, tl, N, phi, force, sys);                               \
+                           item_end, n_items, grid, g, tl, N, phi, force, sys, dense_min);                    \
     } while (0)
     if (phi && force)
         SE_GP(true, true);
@@ -859,15 +865,23 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
                           const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
                           double* phi, double* force, const SysDesc* sys, hipStream_t s)
 {
-#define SE_GPM(PV) interp_pencil_modes<PV>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys, s)
+#define SE_GPM(PV, DM) \
+    interp_pencil_modes<PV>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys, DM, s)
     switch (g.P) {
-    case 8: SE_GPM(8); return true;
-    case 11: SE_GPM(11); return true;
+    case 8: SE_GPM(8, 0); return true;
+    case 11: SE_GPM(11, 0); return true;
     case 16:
-        if (tl.mfma)
-            interp_mfma16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys, s);
-        else
-            SE_GPM(16);
+        if (tl.mfma) {
+            // A 4-particle MFMA group needs about one particle per pencil row; items with
+            // fewer particles than half the tile's (pencil, row) keys (sparse tiles) go to
+            // the vector gather, the others to the MFMA gather.
+            const int dense_min = std::max(1, tl.K / 2);
+            SE_GPM(16, dense_min);
+            interp_mfma16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
+                                dense_min, s);
+        } else {
+            SE_GPM(16, 0);
+        }
         return true;
     default: return false;
     }
