@@ -50,7 +50,7 @@ typedef struct {
     double m;          /* Gaussian shape; 0 selects the driver's 0.95*sqrt(pi*P) */
     int precision;     /* SE_PREC_FP64 or SE_PREC_FP32 (spread, FFT and gather arithmetic) */
     int n_max;         /* particle capacity */
-    int tile;          /* tile side in grid cells, 0 = automatic */
+    int tile;          /* tile side in grid cells (at most 16), 0 = automatic */
     int chunk;         /* max particles per work item, 0 = automatic */
     int block;         /* threads per work-group for spread/gather, 0 = automatic */
     int flags;         /* SE_PLAN_* bits */

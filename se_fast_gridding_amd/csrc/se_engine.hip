@@ -72,14 +72,14 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
         SE_REQUIRE(TS > 0, SE_ERR_ARG, "support P too large for the LDS sub-grid");
     }
     const int Mmin = std::min(M, Mz);
-    TS = std::min(TS, Mmin);
+    TS = std::min(TS, std::min(Mmin, 16));   // the merge keeps a tile's sums in LDS: at most 16^3
     if (tl.pencil) TS -= TS % 4;   // pencils are 4 columns wide in x: tile side a multiple of 4
     if (tl.pencil && TS < 4) {
         tl.pencil = 0;
         tl.mfma = 0;
         TS = std::max(1, std::min(tile > 0 ? tile : Mmin, Mmin));
     }
-    SE_REQUIRE(TS >= 1 && TS <= 255, SE_ERR_ARG, "tile side out of range");
+    SE_REQUIRE(TS >= 1 && TS <= 16, SE_ERR_ARG, "tile side out of range");
     tl.TS = TS;
     tl.NT = (M + TS - 1) / TS;
     tl.NTz = (Mz + TS - 1) / TS;

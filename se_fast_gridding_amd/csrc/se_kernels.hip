@@ -419,38 +419,34 @@ void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* ite
 }
 
 // ================================================================================
-// Halo merge (= wrap), owner-tile form: one work-group per tile of the periodic grid;
-// each thread owns up to EPT points of that tile and sums, in a fixed order, every
-// partial value that lands on them: for each box of partial sub-grid elements that
-// maps onto the tile (the product of one per-dimension segment from each of the three
-// segment tables) and each work item of the box's source tile.  Control flow and the
-// tables are uniform across the work-group, partial reads are z-contiguous, and the
-// per-point order is fixed, so the result does not depend on scheduling.  FP32 partials
-// are summed in FP64; FP64 partials with Kahan compensation.
+// Halo merge (= wrap), owner-tile form: one work-group per tile of the periodic grid
+// sums, in a fixed order, every partial value that lands on the tile: for each box of
+// partial sub-grid elements that maps onto the tile (the product of one per-dimension
+// segment from each of the three segment tables) and each work item of the box's source
+// tile.  The box is read row by row (16 lanes per z run, 16 runs per pass), so the loads
+// are z-contiguous and no lane tests points outside the box; the sums live in an LDS
+// accumulator per tile point, and a barrier after each (box, item) keeps the per-point
+// order fixed, so the result does not depend on scheduling.  FP32 partials are summed in
+// FP64; FP64 partials with Kahan compensation.
 // seg[t * sstride + a] = (source tile coordinate, local start, output start, length).
 // ================================================================================
 template <typename T>
-struct MergeAcc;
+__device__ __forceinline__ void merge_add(double* acc, double* cmp, int o, T v);
 template <>
-struct MergeAcc<float> {
-    double s = 0.0;
-    __device__ void add(float v) { s += (double)v; }
-    __device__ float get() const { return (float)s; }
-};
+__device__ __forceinline__ void merge_add<float>(double* acc, double*, int o, float v)
+{
+    acc[o] += (double)v;
+}
 template <>
-struct MergeAcc<double> {
-    double s = 0.0, c = 0.0;
-    __device__ void add(double v)
-    {
-        double y = v - c;
-        double t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ double get() const { return s; }
-};
+__device__ __forceinline__ void merge_add<double>(double* acc, double* cmp, int o, double v)
+{
+    const double y = v - cmp[o];
+    const double t = acc[o] + y;
+    cmp[o] = (t - acc[o]) - y;
+    acc[o] = t;
+}
 
-template <typename T, int EPT>
+template <typename T>
 __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ partials,
                                                      const int* __restrict__ chunk_start,
                                                      const int4* __restrict__ seg, const int* __restrict__ nseg,
@@ -473,17 +469,15 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
     const int ox = tx * TS, oy = ty * TS, oz = tz * TS;
     const int nx = min(TS, M - ox), ny = min(TS, M - oy), nz = min(TS, Mz - oz);
     const int V = nx * ny * nz;
-    int ex[EPT], ey[EPT], ez[EPT];
-    bool own[EPT];
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-        const int e = threadIdx.x + 256 * k;
-        own[k] = e < V;
-        ez[k] = e % nz;
-        ey[k] = (e / nz) % ny;
-        ex[k] = e / (nz * ny);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* acc = reinterpret_cast<double*>(smem_raw);   // [x][y][z] of the tile
+    double* cmp = acc + TS * TS * TS;                    // Kahan compensation (FP64 partials)
+    for (int e = threadIdx.x; e < V; e += 256) {
+        acc[e] = 0.0;
+        if (sizeof(T) == 8) cmp[e] = 0.0;
     }
-    MergeAcc<T> acc[EPT];
+    __syncthreads();
+    const int lz = threadIdx.x & 15, lr = threadIdx.x >> 4;   // z lane, row lane
     const int na = nseg[tx], nb = nseg[ty], nc = nsegz[tz];
     for (int a = 0; a < na; ++a) {
         const int4 A = seg[tx * sstride + a];
@@ -492,23 +486,25 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
             for (int c = 0; c < nc; ++c) {
                 const int4 C = segz[tz * sstride + c];
                 const int src = (A.x * NT + B.x) * NTz + C.x;
-                const int i1 = chunk_start[src + 1];
+                const int i1 = chunk_start[src + 1], rows = A.w * B.w;
                 for (int it = chunk_start[src]; it < i1; ++it) {
                     const T* base = partials + (size_t)it * tl.part_elems;
-#pragma unroll
-                    for (int k = 0; k < EPT; ++k) {
-                        const unsigned dx = (unsigned)(ex[k] - A.z), dy = (unsigned)(ey[k] - B.z),
-                                       dz = (unsigned)(ez[k] - C.z);
-                        if (own[k] && dx < (unsigned)A.w && dy < (unsigned)B.w && dz < (unsigned)C.w)
-                            acc[k].add(base[((A.y + (int)dx) * S + (B.y + (int)dy)) * S + (C.y + (int)dz)]);
+                    for (int zz = lz; zz < C.w; zz += 16) {   // runs are at most the tile side (<= 16)
+                        for (int rr = lr; rr < rows; rr += 16) {
+                            const int dx = rr / B.w, dy = rr - dx * B.w;
+                            const T v = base[((A.y + dx) * S + (B.y + dy)) * S + (C.y + zz)];
+                            merge_add<T>(acc, cmp, ((A.z + dx) * ny + (B.z + dy)) * nz + (C.z + zz), v);
+                        }
                     }
+                    __syncthreads();
                 }
             }
         }
     }
-#pragma unroll
-    for (int k = 0; k < EPT; ++k)
-        if (own[k]) grid[((long)(ox + ex[k]) * M + (oy + ey[k])) * Mz + (oz + ez[k])] = acc[k].get();
+    for (int e = threadIdx.x; e < V; e += 256) {
+        const int ez = e % nz, ey = (e / nz) % ny, ex = e / (nz * ny);
+        grid[((long)(ox + ex) * M + (oy + ey)) * Mz + (oz + ez)] = (T)acc[e];
+    }
 }
 
 template <typename T>
@@ -516,23 +512,12 @@ void launch_merge(const T* partials, const int* chunk_start, const int4* seg, co
                   const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid, int nsys, hipStream_t s)
 {
     if (nsys <= 0) return;
-    const int V = tl.TS * tl.TS * tl.TS;
     const int cols8 = (tl.NT * tl.NT + 7) / 8;
     dim3 gr(8 * cols8 * tl.NTz, nsys), bl(256);
-#define SE_MG(E)                                                                                              \
-    hipLaunchKernelGGL((k_merge_tiles<T, E>), gr, bl, 0, s, partials, chunk_start, seg, nseg, segz, nsegz,    \
-                       sstride, g, tl, grid)
-    if (V <= 256)
-        SE_MG(1);
-    else if (V <= 512)
-        SE_MG(2);
-    else if (V <= 1024)
-        SE_MG(4);
-    else if (V <= 2048)
-        SE_MG(8);
-    else
-        SE_MG(16);
-#undef SE_MG
+    const size_t lds = (size_t)tl.TS * tl.TS * tl.TS * sizeof(double) * (sizeof(T) == 8 ? 2 : 1);
+    allow_lds((const void*)k_merge_tiles<T>, lds);
+    hipLaunchKernelGGL(k_merge_tiles<T>, gr, bl, lds, s, partials, chunk_start, seg, nseg, segz, nsegz, sstride, g,
+                       tl, grid);
 }
 
 // ================================================================================
