@@ -1,8 +1,9 @@
 """Grouped solves (se_solve_group, se_batch_run): several independent systems of one
 geometry in one pass.  Every system's result is checked against the oracle's SE3P
-solve (SE3P, FP64) or the SE2P direct sum, and against the same system solved alone on
-a plain plan; groups hold uneven sizes, empty and one-particle systems, and the
-executor's grouping by geometry and output mode is checked on a mixed job list."""
+solve (SE3P, FP64) or the SE2P direct sum, and (SE3P, SE2P, SE1P) against the same
+system solved alone on a plain plan; groups hold uneven sizes, empty and one-particle
+systems, and the executor's grouping by geometry and output mode is checked on a mixed
+job list."""
 import numpy as np
 import pytest
 import torch
@@ -37,7 +38,7 @@ def to_dev(x, q, force=True):
 
 def solve_alone(x, q, M, P, xi, prec, per=3, force=True):
     N = len(q)
-    plan = se.Plan(M, P, 1.0, xi, max(N, 1), precision=prec, periodicity=per, slab=1.0 if per == 2 else 0.0)
+    plan = se.Plan(M, P, 1.0, xi, max(N, 1), precision=prec, periodicity=per, slab=1.0 if per != 3 else 0.0)
     xt, qt, phi, F = to_dev(x, q, force)
     plan.solve(xt, qt, phi, F)
     torch.cuda.synchronize()
@@ -49,13 +50,13 @@ SIZES = [300, 0, 1, 2000, 57, 4096, 999, 12000]
 
 
 @pytest.mark.parametrize("prec", [se.PREC_FP64, se.PREC_FP32], ids=["fp64", "fp32"])
-@pytest.mark.parametrize("per", [3, 2], ids=["se3p", "se2p"])
+@pytest.mark.parametrize("per", [3, 2, 1], ids=["se3p", "se2p", "se1p"])
 def test_group_equals_each_system_alone(per, prec):
     M, P, xi = 32, 16 if prec == se.PREC_FP32 else 21, 10.0
     print("case group per=%d prec=%d sizes=%s" % (per, prec, SIZES), flush=True)
     systems = [system(N, 40 + i, slab=1.0 if per == 2 else None) for i, N in enumerate(SIZES)]
     dev = [to_dev(x, q) for x, q in systems]
-    plan = se.Plan(M, P, 1.0, xi, sum(SIZES), precision=prec, periodicity=per, slab=1.0 if per == 2 else 0.0,
+    plan = se.Plan(M, P, 1.0, xi, sum(SIZES), precision=prec, periodicity=per, slab=1.0 if per != 3 else 0.0,
                    max_systems=len(SIZES) + 3)
     plan.solve_group(dev)
     torch.cuda.synchronize()
