@@ -97,6 +97,7 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     tl.sub_elems = tl.S * tl.S * tl.Szp;
     tl.part_elems = tl.S * tl.S * tl.S;
     tl.max_items = tl.ntiles * n_sys + n_cap / tl.chunk + 1;
+    tl.gsplit = 1;
     return tl;
 }
 

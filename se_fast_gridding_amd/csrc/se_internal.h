@@ -78,7 +78,70 @@ struct Tiling {
     int nseg;         // max merge segments per tile coordinate (segment-table stride)
     int pencil;       // 1: register-ring pencil kernels (FP32, P in the compiled set)
     int mfma;         // 1: the pencil gather runs on MFMA (P = 16)
+    int gsplit;       // work-groups per work item in the pencil gathers (>= 1)
 };
+
+// Walk over the S^3 points of a tile sub-grid, e = (a S + b) S + c, visited as e = tid,
+// tid + nthr, ...: the digits (a, b, c) of the thread's point advance by the digits of
+// nthr with carries, so no point costs an integer division.
+struct SubWalk {
+    int a, b, c;      // digits of the current point
+    int da, db, dc;   // digits of the step nthr
+    int S;
+    __device__ SubWalk(int S_, int tid, int nthr) : S(S_)
+    {
+        c = tid % S;
+        b = (tid / S) % S;
+        a = tid / (S * S);
+        dc = nthr % S;
+        db = (nthr / S) % S;
+        da = nthr / (S * S);
+    }
+    __device__ void next()
+    {
+        c += dc;
+        const int k1 = c >= S ? 1 : 0;
+        c -= k1 * S;
+        b += db + k1;
+        const int k2 = b >= S ? 1 : 0;
+        b -= k2 * S;
+        a += da + k2;
+    }
+};
+
+// v mod n for 0 <= v < (k + 1) n with small k (sub-grid offsets past the grid end)
+__device__ __forceinline__ int wrap_index(int v, int n)
+{
+    while (v >= n) v -= n;
+    return v;
+}
+
+// Loads the periodic S^3 block of `grid` (M x M x Mz, z fastest) that starts at grid
+// point (ox, oy, oz) into LDS, laid out [y][x][z] (sub[(iy S + ix) S + iz]) if YXZ, else
+// [x][y][z].  Each thread issues U loads before it writes any of them, so U global loads
+// are in flight per thread instead of one.
+template <bool YXZ, typename T, int U = 8>
+__device__ __forceinline__ void load_tile(T* sub, const T* __restrict__ grid, int S, int M, int Mz, int ox, int oy,
+                                          int oz, int tid, int nthr)
+{
+    const int S3 = S * S * S;
+    SubWalk w(S, tid, nthr);   // (a, b) = (iy, ix) if YXZ, else (ix, iy); c = iz
+    for (int e0 = tid; e0 < S3; e0 += U * nthr) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (e0 + u * nthr < S3) {
+                const int ix = YXZ ? w.b : w.a, iy = YXZ ? w.a : w.b;
+                const int gx = wrap_index(ox + ix, M), gy = wrap_index(oy + iy, M), gz = wrap_index(oz + w.c, Mz);
+                v[u] = grid[((long)gx * M + gy) * Mz + gz];
+            }
+            w.next();
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (e0 + u * nthr < S3) sub[e0 + u * nthr] = v[u];
+    }
+}
 
 // Particle record written by the bucketing scatter, read by spread and gather.
 template <typename T>

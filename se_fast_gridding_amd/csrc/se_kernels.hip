@@ -383,9 +383,10 @@ __global__ __launch_bounds__(512) void k_spread_tiles(
     __syncthreads();
     T* out = partials + (size_t)item * tl.part_elems;
     const int S = tl.S;
+    SubWalk w(S, tid, nthr);   // a = ix, b = iy, c = iz
     for (int e = tid; e < tl.part_elems; e += nthr) {
-        int iz = e % S, r2 = e / S, iy = r2 % S, ix = r2 / S;
-        out[e] = (T)sub[(ix * S + iy) * tl.Szp + iz];
+        out[e] = (T)sub[(w.a * S + w.b) * tl.Szp + w.c];
+        w.next();
     }
 }
 
@@ -552,11 +553,8 @@ __global__ __launch_bounds__(512) void k_interp_tiles(
             N = sys[si].N;
         }
         const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
-        for (int e = tid; e < tl.part_elems; e += nthr) {
-            int iz = e % S, r2 = e / S, iy = r2 % S, ix = r2 / S;
-            int gx = (ox + ix) % M, gy = (oy + iy) % M, gz = (oz + iz) % Mz;
-            sub[(ix * S + iy) * tl.Szp + iz] = grid[((long)gx * M + gy) * Mz + gz];
-        }
+        // [x][y][z] with z pitch Szp == S (choose_tiling), so point e lands at sub[e]
+        load_tile<false>(sub, grid, S, M, Mz, ox, oy, oz, tid, nthr);
     }
     __syncthreads();
 

@@ -74,6 +74,16 @@ __device__ __forceinline__ LaneRec load_lane_rec(const Rec<float>* rec, int i)
     return r;
 }
 
+// Waits for a batch's loads at this point: an inline-asm use of every loaded register makes
+// the compiler place the vmcnt wait here.  Without it the wait lands where the batch is
+// next read, which in a loop with a conditional reload is the top of every iteration, and
+// vmcnt also counts the stores issued since, so each iteration would wait for the
+// previous iteration's output stores to complete.
+__device__ __forceinline__ void settle(const LaneRec& r)
+{
+    asm volatile("" ::"v"(r.a.x), "v"(r.a.y), "v"(r.a.z), "v"(r.a.w), "v"(r.b.x), "v"(r.b.y));
+}
+
 __device__ __forceinline__ Rec<float> pick_rec(const LaneRec& s, int l)
 {
     Rec<float> r;
@@ -337,9 +347,10 @@ __global__ __launch_bounds__(512) void k_spread_pencil(const Rec<float>* __restr
     if (pen >= 0) flush(base + P);
     __syncthreads();
     float* out = partials + (size_t)item * tl.part_elems;
+    SubWalk w(S, tid, nthr);   // canonical [x][y][z]: a = ix, b = iy, c = iz
     for (int e = tid; e < tl.part_elems; e += nthr) {
-        const int iz = e % S, r2 = e / S, iy = r2 % S, ix = r2 / S;   // canonical [x][y][z]
-        out[e] = (float)sub[(iy * S + ix) * S + iz];
+        out[e] = (float)sub[(w.b * S + w.a) * S + w.c];
+        w.next();
     }
 }
 
@@ -354,10 +365,15 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
                                                        const SysDesc* __restrict__ sys, int dense_min)
 {
     constexpr int R = ((P + 3) * P + 63) / 64;
-    const int item = blockIdx.x;
+    // tl.gsplit work-groups share an item, each taking an equal part of its particles
+    const int item = blockIdx.x / tl.gsplit, part = blockIdx.x - item * tl.gsplit;
     if (item >= *n_items) return;
+    const int ibeg = item_beg[item], iend = item_end[item];
     // items of dense_min particles or more are the MFMA gather's (dense_min > 0)
-    if (dense_min > 0 && item_end[item] - item_beg[item] >= dense_min) return;
+    if (dense_min > 0 && iend - ibeg >= dense_min) return;
+    const int beg = ibeg + (int)(((long)(iend - ibeg) * part) / tl.gsplit);
+    const int end = ibeg + (int)(((long)(iend - ibeg) * (part + 1)) / tl.gsplit), n = end - beg;
+    if (n == 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -373,18 +389,13 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
             N = sys[si].N;
         }
         const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
-        for (int e = tid; e < S3; e += nthr) {
-            const int iz = e % S, r2 = e / S, ix = r2 % S, iy = r2 / S;
-            const int gx = (ox + ix) % M, gy = (oy + iy) % M, gz = (oz + iz) % Mz;
-            sub[e] = grid[((long)gx * M + gy) * Mz + gz];
-        }
+        load_tile<true>(sub, grid, S, M, Mz, ox, oy, oz, tid, nthr);
     }
     __syncthreads();
 
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
     int pi[R], pk[R], pofs[R], bx[R], bz[R];
     lane_pairs<P, R>(lane, S, pi, pk, pofs, bx, bz);
-    const int beg = item_beg[item], end = item_end[item], n = end - beg;
     const int wb = beg + (int)(((long)n * wave) / nw), we = beg + (int)(((long)n * (wave + 1)) / nw);
     const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
     const double h3 = g.h * g.h * g.h;
@@ -551,9 +562,10 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
 // also the (at most 4) rows and (3) columns outside a particle's P-point window: those
 // lie at least 8.5 h from the particle, where the Gaussian is below e^{-c 72 h^2}
 // (1e-10 relative at the P = 16 shape), far below FP32 resolution, so no masks are
-// needed.  A factor depends only on the particle slot j and the row or column, so each is
-// evaluated once per group (on lane 4 s + j) and fetched across the 16 blocks with
-// ds_bpermute instead of being recomputed on every block.
+// needed.  A y factor reaches all 16 blocks through the MFMA's B lane-group broadcast
+// (each lane evaluates 5 of them); an x factor depends only on the particle slot j and
+// the column, so each is evaluated once per group (on lane 4 c + j) and fetched across
+// the blocks with ds_bpermute, issued ahead of the group's MFMAs.
 // ---------------------------------------------------------------------------------
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -568,6 +580,18 @@ __device__ __forceinline__ float sum_over_blocks(float v)
     v = __uint_as_float(a[0]) + __uint_as_float(a[1]);
     const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
     return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+
+// D += A B with the B operand of 16-lane group g (lanes 16g .. 16g + 15) broadcast to all
+// four groups (BLGP = 4 + g); g is a constant after unrolling.
+__device__ __forceinline__ v4f mfma4_bcast(float a, float b, v4f c, int g)
+{
+    switch (g) {
+    case 0: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 4);
+    case 1: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 5);
+    case 2: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 6);
+    default: return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 7);
+    }
 }
 
 // Per-lane data of one group (this lane's particle: slot lane & 3 of the group).
@@ -592,10 +616,15 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
                                                        const SysDesc* __restrict__ sys, int dense_min)
 {
     constexpr int P = 16, GY = 20, R = 5;
-    const int item = blockIdx.x;
+    // tl.gsplit work-groups share an item, each taking an equal part of its particles
+    const int item = blockIdx.x / tl.gsplit, part = blockIdx.x - item * tl.gsplit;
     if (item >= *n_items) return;
+    const int ibeg = item_beg[item], iend = item_end[item];
     // items below dense_min particles are left to the vector gather
-    if (item_end[item] - item_beg[item] < dense_min) return;
+    if (iend - ibeg < dense_min) return;
+    const int beg = ibeg + (int)(((long)(iend - ibeg) * part) / tl.gsplit);
+    const int end = ibeg + (int)(((long)(iend - ibeg) * (part + 1)) / tl.gsplit), n = end - beg;
+    if (n == 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -611,17 +640,13 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
             N = sys[si].N;
         }
         const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
-        for (int e = tid; e < S3; e += nthr) {
-            const int iz = e % S, r2 = e / S, ix = r2 % S, iy = r2 / S;
-            const int gx = (ox + ix) % M, gy = (oy + iy) % M, gz = (oz + iz) % Mz;
-            sub[e] = grid[((long)gx * M + gy) * Mz + gz];
-        }
+        load_tile<true>(sub, grid, S, M, Mz, ox, oy, oz, tid, nthr);
     }
     __syncthreads();
 
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
     const int blk = lane >> 2, q4 = lane & 3;   // block = z column; A-row = particle slot
-    const int beg = item_beg[item], end = item_end[item], n = end - beg;
+    const int lgrp = lane >> 4;                  // 16-lane group
     const int wb = beg + (int)(((long)n * wave) / nw), we = beg + (int)(((long)n * (wave + 1)) / nw);
     const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
     const float kc = -c * 1.4426950408889634f;
@@ -659,12 +684,14 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
 
     int b0 = wb;
     LaneRec batch = load_lane_rec(rec, min(b0 + lane, max(we - 1, beg)));
+    settle(batch);
     // the next group from particle p0 (< we): up to 4 particles of p0's pencil whose rows lie
     // in [y0, y0 + GY - P]; moves the ring and returns this lane's particle data
     auto next_group = [&](int p0, int& k) -> GroupLane {
         if (p0 + 4 > b0 + 64) {   // records p0 .. p0 + 3 must lie in the batch
             b0 = p0;
             batch = load_lane_rec(rec, min(b0 + lane, we - 1));
+            settle(batch);
         }
         const int lo0 = __builtin_amdgcn_readlane(batch.b.x, p0 - b0);
         const int lx0 = lo0 & 255, ly0 = (lo0 >> 8) & 255, lz0 = lo0 >> 16;
@@ -704,51 +731,52 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
         return d;
     };
 
-    // y contraction of the group in `d` over the 20 ring rows into (Dp, Df).  A y factor
-    // depends only on (particle slot j, ring slot s), so lane l forms the factors of
-    // (j = l & 3, s = l >> 2) and (j, s = 16 + (l >> 2)) once, and each B operand column is
-    // fetched with a ds_bpermute (source lane 4 s + j); the MFMAs then issue back to back.
+    // y contraction of the group in `d` over the 20 ring rows into (Dp, Df).  The B
+    // operand of ring slot s = 4u + v is register u of the y factors with lane group v
+    // broadcast to all four 16-lane groups (MFMA BLGP = 4 + v): lane 16v + 4c + j of
+    // register u holds particle j's y factor at the row of slot 4u + v (the same value for
+    // c = 0..3), so every block sees Y_j of that row in column j.  Each lane evaluates 5
+    // exponentials and no cross-lane fetch sits between the MFMAs.
     auto contract = [&](const GroupLane& d, v4f (&Dp)[R], v4f (&Df)[R]) {
         const v4f zero4 = {0.f, 0.f, 0.f, 0.f};
         const int bm = base % GY;
-        const int s1 = blk, s2 = min(16 + blk, GY - 1);
-        const int rho1 = base + (s1 >= bm ? s1 - bm : s1 - bm + GY);
-        const int rho2 = base + (s2 >= bm ? s2 - bm : s2 - bm + GY);
-        const float ay1 = fmaf(-(float)rho1, h, d.tyl), ay2 = fmaf(-(float)rho2, h, d.tyl);
-        const float y1 = __builtin_amdgcn_exp2f(kc * (ay1 * ay1)), y2 = __builtin_amdgcn_exp2f(kc * (ay2 * ay2));
-        const float b1 = y1 * ay1, b2 = y2 * ay2;
-        const int src = 4 * q4;
-        float yv[GY], yb[GY];
+        float yv[GY / 4], yb[GY / 4];
 #pragma unroll
-        for (int s = 0; s < GY; ++s) {
-            yv[s] = bperm(src + 16 * (s & 15), s < 16 ? y1 : y2);
-            if (FORCE) yb[s] = bperm(src + 16 * (s & 15), s < 16 ? b1 : b2);
+        for (int u = 0; u < GY / 4; ++u) {
+            const int t = 4 * u + lgrp - bm;
+            const int rho = base + (t < 0 ? t + GY : t);
+            const float ay = fmaf(-(float)rho, h, d.tyl);
+            yv[u] = __builtin_amdgcn_exp2f(kc * (ay * ay));
+            if (FORCE) yb[u] = yv[u] * ay;
         }
 #pragma unroll
         for (int s = 0; s < GY; ++s) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                Dp[r] = __builtin_amdgcn_mfma_f32_4x4x1f32(G[s][r], yv[s], s == 0 ? zero4 : Dp[r], 0, 0, 0);
-                if (FORCE)
-                    Df[r] = __builtin_amdgcn_mfma_f32_4x4x1f32(G[s][r], yb[s], s == 0 ? zero4 : Df[r], 0, 0, 0);
+                Dp[r] = mfma4_bcast(G[s][r], yv[s >> 2], s == 0 ? zero4 : Dp[r], s & 3);
+                if (FORCE) Df[r] = mfma4_bcast(G[s][r], yb[s >> 2], s == 0 ? zero4 : Df[r], s & 3);
             }
         }
     };
 
-    // x and z factors of this lane's columns, sums over the blocks, outputs of the group in d
-    auto epilogue = [&](const GroupLane& d, const v4f (&Dp)[R], const v4f (&Df)[R]) {
-        // x factors of (j = l & 3, column l >> 2) and (j, column 16 + (l >> 2)) on lane l,
-        // fetched per column like the y factors
+    // x factors of (j = l & 3, column l >> 2) and (j, column 16 + (l >> 2)) on lane l,
+    // fetched per column like a table (source lane 4 c + j); issued before the group's
+    // MFMAs so that the fetch latency hides behind them
+    auto xfactors = [&](const GroupLane& d, float (&xv)[4 * R], float (&xa)[4 * R]) {
         const float ax1 = fmaf(d.oxf - (float)blk, h, d.txf), ax2 = fmaf(d.oxf - (float)(16 + blk), h, d.txf);
         const float x1 = __builtin_amdgcn_exp2f(kc * (ax1 * ax1)), x2 = __builtin_amdgcn_exp2f(kc * (ax2 * ax2));
         const float xa1 = x1 * ax1, xa2 = x2 * ax2;
         const int src = 4 * q4;
-        float xv[4 * R], xa[4 * R];
 #pragma unroll
         for (int c4 = 0; c4 < 4 * R; ++c4) {
             xv[c4] = bperm(src + 16 * (c4 & 15), c4 < 16 ? x1 : x2);
             if (FORCE) xa[c4] = bperm(src + 16 * (c4 & 15), c4 < 16 ? xa1 : xa2);
         }
+    };
+
+    // x and z factors of this lane's columns, sums over the blocks, outputs of the group in d
+    auto epilogue = [&](const GroupLane& d, const v4f (&Dp)[R], const v4f (&Df)[R], const float (&xv)[4 * R],
+                        const float (&xa)[4 * R]) {
         float S0 = 0.f, S1 = 0.f, S2 = 0.f;
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -782,10 +810,13 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
     };
 
     v4f Dp[R], Df[R];
+    float xv[4 * R], xa[4 * R];
     for (int p0 = wb, k = 0; p0 < we; p0 += k) {
         const GroupLane d = next_group(p0, k);
+        xfactors(d, xv, xa);
+        __builtin_amdgcn_sched_barrier(0);   // the fetches stay ahead of the MFMA block
         contract(d, Dp, Df);
-        epilogue(d, Dp, Df);
+        epilogue(d, Dp, Df, xv, xa);
     }
 }
 
@@ -822,7 +853,7 @@ static void interp_mfma16_modes(const Rec<float>* rec, const int* item_tile, con
                                 int dense_min, hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
-    dim3 gr(tl.max_items), bl(tl.block);
+    dim3 gr(tl.max_items * tl.gsplit), bl(tl.block);
 #define SE_GM(POT, FORCE)                                                                                     \
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_mfma16<POT, FORCE>, lds);                                           \
@@ -845,7 +876,7 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
                                 int dense_min, hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
-    dim3 gr(tl.max_items), bl(tl.block);
+    dim3 gr(tl.max_items * tl.gsplit), bl(tl.block);
 #define SE_GP(POT, FORCE)                                                                                     \
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_pencil<PV, POT, FORCE>, lds);                                       \
