@@ -97,7 +97,11 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     tl.sub_elems = tl.S * tl.S * tl.Szp;
     tl.part_elems = tl.S * tl.S * tl.S;
     tl.max_items = tl.ntiles * n_sys + n_cap / tl.chunk + 1;
-    tl.gsplit = 1;
+    // The MFMA gather holds an FP32 sub-grid (half the spread's), so two work-groups of
+    // at most 256 threads share a CU and one's tile load overlaps the other's particles;
+    // it runs two waves per SIMD either way, limited by registers.  (The vector gathers
+    // measured fastest at 512.)
+    tl.mblock = std::min(tl.block, 256);
     return tl;
 }
 

@@ -78,7 +78,7 @@ struct Tiling {
     int nseg;         // max merge segments per tile coordinate (segment-table stride)
     int pencil;       // 1: register-ring pencil kernels (FP32, P in the compiled set)
     int mfma;         // 1: the pencil gather runs on MFMA (P = 16)
-    int gsplit;       // work-groups per work item in the pencil gathers (>= 1)
+    int mblock;       // threads per work-group of the MFMA gather
 };
 
 // Walk over the S^3 points of a tile sub-grid, e = (a S + b) S + c, visited as e = tid,

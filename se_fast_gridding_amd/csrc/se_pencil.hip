@@ -365,15 +365,11 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
                                                        const SysDesc* __restrict__ sys, int dense_min)
 {
     constexpr int R = ((P + 3) * P + 63) / 64;
-    // tl.gsplit work-groups share an item, each taking an equal part of its particles
-    const int item = blockIdx.x / tl.gsplit, part = blockIdx.x - item * tl.gsplit;
+    const int item = blockIdx.x;
     if (item >= *n_items) return;
-    const int ibeg = item_beg[item], iend = item_end[item];
+    const int beg = item_beg[item], end = item_end[item], n = end - beg;
     // items of dense_min particles or more are the MFMA gather's (dense_min > 0)
-    if (dense_min > 0 && iend - ibeg >= dense_min) return;
-    const int beg = ibeg + (int)(((long)(iend - ibeg) * part) / tl.gsplit);
-    const int end = ibeg + (int)(((long)(iend - ibeg) * (part + 1)) / tl.gsplit), n = end - beg;
-    if (n == 0) return;
+    if (dense_min > 0 && n >= dense_min) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -616,15 +612,11 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
                                                        const SysDesc* __restrict__ sys, int dense_min)
 {
     constexpr int P = 16, GY = 20, R = 5;
-    // tl.gsplit work-groups share an item, each taking an equal part of its particles
-    const int item = blockIdx.x / tl.gsplit, part = blockIdx.x - item * tl.gsplit;
+    const int item = blockIdx.x;
     if (item >= *n_items) return;
-    const int ibeg = item_beg[item], iend = item_end[item];
+    const int beg = item_beg[item], end = item_end[item], n = end - beg;
     // items below dense_min particles are left to the vector gather
-    if (iend - ibeg < dense_min) return;
-    const int beg = ibeg + (int)(((long)(iend - ibeg) * part) / tl.gsplit);
-    const int end = ibeg + (int)(((long)(iend - ibeg) * (part + 1)) / tl.gsplit), n = end - beg;
-    if (n == 0) return;
+    if (n < dense_min) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -853,7 +845,7 @@ static void interp_mfma16_modes(const Rec<float>* rec, const int* item_tile, con
                                 int dense_min, hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
-    dim3 gr(tl.max_items * tl.gsplit), bl(tl.block);
+    dim3 gr(tl.max_items), bl(tl.mblock);
 #define SE_GM(POT, FORCE)                                                                                     \
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_mfma16<POT, FORCE>, lds);                                           \
@@ -876,7 +868,7 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
                                 int dense_min, hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
-    dim3 gr(tl.max_items * tl.gsplit), bl(tl.block);
+    dim3 gr(tl.max_items), bl(tl.block);
 #define SE_GP(POT, FORCE)                                                                                     \
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_pencil<PV, POT, FORCE>, lds);                                       \
