@@ -801,14 +801,26 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
         }
     };
 
+    // One group ahead: the next group's ring rows and record fields are fetched after the
+    // current group's MFMAs have read the ring and before its epilogue, whose work then
+    // covers the LDS latency of those fetches.
     v4f Dp[R], Df[R];
     float xv[4 * R], xa[4 * R];
-    for (int p0 = wb, k = 0; p0 < we; p0 += k) {
-        const GroupLane d = next_group(p0, k);
+    int p0 = wb, k = 0;
+    GroupLane d{};
+    if (p0 < we) d = next_group(p0, k);
+    while (p0 < we) {
         xfactors(d, xv, xa);
         __builtin_amdgcn_sched_barrier(0);   // the fetches stay ahead of the MFMA block
         contract(d, Dp, Df);
+        const int pn = p0 + k;
+        int kn = 0;
+        GroupLane dn{};
+        if (pn < we) dn = next_group(pn, kn);
         epilogue(d, Dp, Df, xv, xa);
+        d = dn;
+        p0 = pn;
+        k = kn;
     }
 }
 
