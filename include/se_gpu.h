@@ -52,7 +52,8 @@ typedef struct {
     int n_max;         /* particle capacity */
     int tile;          /* tile side in grid cells (at most 16), 0 = automatic */
     int chunk;         /* max particles per work item, 0 = automatic */
-    int block;         /* threads per work-group for spread/gather, 0 = automatic */
+    int block;         /* threads per work-group of the spread and the vector gathers (the P = 16
+                          MFMA gather uses at most 256), 0 = automatic */
     int flags;         /* SE_PLAN_* bits */
     double slab;       /* SE2P: particles have 0 <= z < slab; SE1P: particles have
                           0 <= x, y < slab (ignored for SE3P) */
@@ -187,6 +188,24 @@ int se_tune_save(const char* path);
 int se_tune_load(const char* path);
 int se_tune_clear(void);
 int se_tune_lookup(const se_plan_desc* desc, int N, int out[3]);
+
+/*
+ * Parameters for an accuracy target (host only).  Applies the error estimates that
+ * SE_init_params (SE_general.c:517-572) prints -- it replaces no entry point of the
+ * reference, which takes P from the caller -- to choose the smallest support P <= 32
+ * whose estimated rms gridding error for `quantities` (SE_Q_POTENTIAL and/or SE_Q_FORCE;
+ * the larger estimate when both) is at most tol, for a cubic box of side L, Ewald
+ * parameter xi and Q = sum q^2:
+ *   potential: sqrt(Q xi L) / L * exp(-pi P c^2 / 2)                 (SE_general.c:530)
+ *   force:     Q sqrt(xi L) / L^2 * exp(-pi P c^2 / 2) * 2 pi c^2 P  (SE_general.c:544)
+ * with c = 0.95.  *m = c sqrt(pi P) (the Gaussian shape), *M = the reference's grid
+ * (k_inf = 2 xi L, M = ceil(2 k_inf + 1) + 2, at least P, rounded up to a multiple of
+ * 4), *est (optional) = the estimate at P.  The FP32 register-ring kernels cover
+ * P in {8, 11, 16}; other P run the generic kernels.  SE_ERR_ARG when no P <= 32 meets
+ * tol or an argument is out of range.
+ */
+int se_select_params(double tol, double L, double xi, double Q, int quantities, int* P, int* M, double* m,
+                     double* est);
 
 /*
  * Real-space and self terms of the Ewald sum, so that real + k-space (se_solve) + self

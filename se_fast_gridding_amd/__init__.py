@@ -187,6 +187,8 @@ _SIG = {
     "se_tune_load": (C.c_int, [C.c_char_p]),
     "se_tune_clear": (C.c_int, []),
     "se_tune_lookup": (C.c_int, [C.POINTER(se_plan_desc), C.c_int, _I]),
+    "se_select_params": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _I, _I,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 }
 
 _SPREAD_SPLIT = ["SE_FGG_grid_split_SSE_dispatch", "SE_FGG_grid_split", "SE_FGG_grid_split_SSE",
@@ -386,6 +388,24 @@ def tune_lookup(M, P, box, xi, N, precision=PREC_FP32, periodicity=3, slab=0.0):
     d = se_plan_desc(periodicity, M, P, float(box), float(xi), 0.0, precision, int(N), 0, 0, 0, 0, float(slab), 0)
     out = (C.c_int * 3)()
     return tuple(out) if capi().se_tune_lookup(C.byref(d), int(N), out) == 1 else None
+
+
+def select_params(tol, L, xi, q, potential=True, force=False):
+    """(P, M, m, estimate) for an rms error target (se_gpu.h se_select_params): the
+    smallest support P whose estimated gridding error (the estimates SE_init_params
+    prints, SE_general.c:525-549) is at most tol, with the reference's grid rule for M.
+    q is the charge array (Q = sum q^2) or Q itself as a float."""
+    import numpy as np
+
+    Q = float(q) if np.ndim(q) == 0 else float(np.sum(np.asarray(q, dtype=np.float64) ** 2))
+    P, M = C.c_int(), C.c_int()
+    m, est = C.c_double(), C.c_double()
+    qty = (1 if potential else 0) | (2 if force else 0)
+    rc = capi().se_select_params(float(tol), float(L), float(xi), Q, qty, C.byref(P), C.byref(M), C.byref(m),
+                                 C.byref(est))
+    if rc != SE_OK:
+        raise ValueError("se_select_params: no support P <= 32 meets tol=%g (or an argument is out of range)" % tol)
+    return P.value, M.value, m.value, est.value
 
 
 class RealSpace:

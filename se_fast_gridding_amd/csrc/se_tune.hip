@@ -249,4 +249,33 @@ int se_tune_lookup(const se_plan_desc* desc, int N, int out[3])
     return tuned_tiling(desc, N, out[0], out[1], out[2]) ? 1 : 0;
 }
 
+// Parameter selection for an accuracy target, from the estimates SE_init_params prints
+// (SE_general.c:525-549): Lindbo-Tornberg gridding errors with c = 0.95, and the
+// reference's grid rule (SE_general.c:536-563, AVX build).
+int se_select_params(double tol, double L, double xi, double Q, int quantities, int* P, int* M, double* m,
+                     double* est)
+{
+    if (!(tol > 0) || !(L > 0) || !(xi > 0) || !(Q >= 0) || !P || !M || !m) return SE_ERR_ARG;
+    if (!(quantities & (SE_Q_POTENTIAL | SE_Q_FORCE))) return SE_ERR_ARG;
+    const double c = 0.95, PI = 3.14159265358979323846;
+    auto estimate = [&](int p) {
+        const double g = std::exp(-PI * p * c * c / 2.);
+        double e = 0.;
+        if (quantities & SE_Q_POTENTIAL) e = std::max(e, std::sqrt(Q * xi * L) / L * g);
+        if (quantities & SE_Q_FORCE) e = std::max(e, Q * std::sqrt(xi * L) / (L * L) * g * 2. * PI * c * c * p);
+        return e;
+    };
+    int p = 1;
+    while (p <= 32 && estimate(p) > tol) ++p;
+    if (p > 32) return SE_ERR_ARG;
+    int Ml = (int)std::ceil(2. * (2. * xi * L) + 1) + 2;
+    if (Ml < p) Ml = p;
+    if (Ml % 4) Ml += 4 - Ml % 4;
+    *P = p;
+    *M = Ml;
+    *m = c * std::sqrt(PI * p);
+    if (est) *est = estimate(p);
+    return SE_OK;
+}
+
 }  // extern "C"
