@@ -447,6 +447,8 @@ __device__ __forceinline__ void merge_add<double>(double* acc, double* cmp, int 
     acc[o] = t;
 }
 
+constexpr int MERGE_U = 8;   // partial rows in flight per thread in the merge
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ partials,
                                                      const int* __restrict__ chunk_start,
@@ -488,17 +490,32 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
                 const int4 C = segz[tz * sstride + c];
                 const int src = (A.x * NT + B.x) * NTz + C.x;
                 const int i1 = chunk_start[src + 1], rows = A.w * B.w;
-                for (int it = chunk_start[src]; it < i1; ++it) {
-                    const T* base = partials + (size_t)it * tl.part_elems;
-                    for (int zz = lz; zz < C.w; zz += 16) {   // runs are at most the tile side (<= 16)
-                        for (int rr = lr; rr < rows; rr += 16) {
-                            const int dx = rr / B.w, dy = rr - dx * B.w;
-                            const T v = base[((A.y + dx) * S + (B.y + dy)) * S + (C.y + zz)];
-                            merge_add<T>(acc, cmp, ((A.z + dx) * ny + (B.z + dy)) * nz + (C.z + zz), v);
+                // A point of this segment triple belongs to one thread (z run lz, row lr +
+                // 16 j; runs are at most the tile side, <= 16), for every item of the box,
+                // so the items need no barrier between them; the barrier after the triple
+                // orders the triples.  U loads are issued before their adds.
+                if (lz < C.w) {
+                    for (int it = chunk_start[src]; it < i1; ++it) {
+                        const T* base = partials + (size_t)it * tl.part_elems + C.y + lz;
+                        for (int r0 = lr; r0 < rows; r0 += 16 * MERGE_U) {
+                            T v[MERGE_U];
+                            int o[MERGE_U];
+#pragma unroll
+                            for (int u = 0; u < MERGE_U; ++u) {
+                                const int rr = r0 + 16 * u;
+                                if (rr < rows) {
+                                    const int dx = rr / B.w, dy = rr - dx * B.w;
+                                    v[u] = base[((A.y + dx) * S + (B.y + dy)) * S];
+                                    o[u] = ((A.z + dx) * ny + (B.z + dy)) * nz + C.z + lz;
+                                }
+                            }
+#pragma unroll
+                            for (int u = 0; u < MERGE_U; ++u)
+                                if (r0 + 16 * u < rows) merge_add<T>(acc, cmp, o[u], v[u]);
                         }
                     }
-                    __syncthreads();
                 }
+                __syncthreads();
             }
         }
     }
