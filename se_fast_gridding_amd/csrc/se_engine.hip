@@ -109,6 +109,25 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
 
 using namespace se;
 
+namespace se {
+
+// Exclusive scan of the per-key counts, read in key order from their counter slots
+// (count_slot).
+struct SlotCount {
+    const int* c;
+    int n;
+    __host__ __device__ int operator()(int k) const { return c[count_slot(k, n)]; }
+};
+
+static hipError_t scan_counts(void* tmp, size_t& bytes, const int* count, int* out, int n, hipStream_t s)
+{
+    hipcub::TransformInputIterator<int, SlotCount, hipcub::CountingInputIterator<int>> it(
+        hipcub::CountingInputIterator<int>(0), SlotCount{count, n});
+    return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, it, out, n, s);
+}
+
+}  // namespace se
+
 struct se_plan {
     se_plan_desc desc{};
     int esize = 8;
@@ -453,7 +472,7 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int
     else
         launch_bin_keys<T>(x, N, pl->g, tl, pl->key, pl->rank, pl->count, s);
     size_t tb = pl->cub_bytes;
-    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->count, pl->row_start, (int)(nb + 1), s));
+    SE_HIP(scan_counts(pl->cub_tmp, tb, pl->count, pl->row_start, (int)(nb + 1), s));
     launch_tile_chunks(pl->row_start, nt, tl.K, tl.chunk, pl->tile_count, pl->nchunk, s);
     tb = pl->cub_bytes;
     SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->nchunk, pl->chunk_start, nt + 1, s));
@@ -685,8 +704,8 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         pl->grid = pl->alloc(ns * M * M * Mz * pl->esize);
         pl->spec = pl->alloc(ns * M * M * (Mz / 2 + 1) * 2 * pl->esize);
         pl->gtab = (double*)pl->alloc((pl->g.per == 3 ? 2 * M + 2 * Mz + 1 : M * M * (Mz / 2 + 1)) * sizeof(double));
-        SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->count, pl->row_start,
-                                                (int)(ns * tl.nbins + 1), pl->stream));
+        SE_HIP(scan_counts(nullptr, pl->cub_bytes, pl->count, pl->row_start, (int)(ns * tl.nbins + 1),
+                           pl->stream));
         if (ns > 1) {
             pl->d_sys = (SysDesc*)pl->alloc(ns * sizeof(SysDesc));
             for (int i = 0; i < 4; ++i) {
@@ -975,7 +994,8 @@ int se_plan_info(se_plan* pl, int out[8])
         SE_HIP(hipStreamSynchronize(pl->stream));
         int ni = 0, bad = 0;
         SE_HIP(hipMemcpy(&ni, pl->n_items, sizeof(int), hipMemcpyDeviceToHost));
-        SE_HIP(hipMemcpy(&bad, pl->count + (long)pl->last_S * pl->tl.nbins, sizeof(int), hipMemcpyDeviceToHost));
+        const int nb = pl->last_S * pl->tl.nbins;   // the discard bucket's key
+        SE_HIP(hipMemcpy(&bad, pl->count + count_slot(nb, nb + 1), sizeof(int), hipMemcpyDeviceToHost));
         out[4] = ni;
         out[5] = bad;
         out[6] = pl->tl.S;

@@ -143,6 +143,18 @@ __device__ __forceinline__ void load_tile(T* sub, const T* __restrict__ grid, in
     }
 }
 
+// Counter slot of sort key k (nslots = keys + 1, the discard bucket included).  The
+// particles of a dense cluster fall into runs of consecutive keys; their counters are
+// scattered over blocks of 2^16 slots (256 KB; an odd multiplier is a bijection mod
+// 2^16) so that the bucketing atomics of one cluster do not queue on one memory
+// channel.  Keys in the last, partial block keep their slot.  The scan of the counts
+// reads them back in key order through the same map.
+__host__ __device__ __forceinline__ int count_slot(int k, int nslots)
+{
+    const int hi = k & ~0xFFFF;
+    return hi + 0x10000 <= nslots ? (hi | (int)(((unsigned)k * 40503u) & 0xFFFFu)) : k;
+}
+
 // Particle record written by the bucketing scatter, read by spread and gather.
 template <typename T>
 struct alignas(16) Rec {

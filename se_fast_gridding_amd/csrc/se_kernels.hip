@@ -147,7 +147,7 @@ __global__ void k_bin_keys(const double* __restrict__ x, int N, Geom g, Tiling t
     // discard bucket after all tiles
     int k = (ok0 && ok1 && ok2) ? row_key(w0, w1, w2, tl) : tl.nbins;
     key[n] = k;
-    rank[n] = atomicAdd(&count[k], 1);
+    rank[n] = atomicAdd(&count[count_slot(k, tl.nbins + 1)], 1);
 }
 
 template <typename T>
@@ -264,7 +264,7 @@ __global__ void k_bin_keys_group(const SysDesc* __restrict__ sys, int S, Geom g,
     const int w2 = window_start_z(d.x[n + 2l * d.N], g, t, ok2);
     const int k = (ok0 && ok1 && ok2) ? si * tl.nbins + row_key(w0, w1, w2, tl) : S * tl.nbins;
     key[d.off + n] = k;
-    rank[d.off + n] = atomicAdd(&count[k], 1);
+    rank[d.off + n] = atomicAdd(&count[count_slot(k, S * tl.nbins + 1)], 1);
 }
 
 template <typename T>
