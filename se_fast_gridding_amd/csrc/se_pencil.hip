@@ -909,9 +909,14 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
         if (tl.mfma) {
             // A 4-particle MFMA group needs about one particle per pencil row; items with
             // fewer particles than half the tile's (pencil, row) keys (sparse tiles) go to
-            // the vector gather, the others to the MFMA gather.
+            // the vector gather, the others to the MFMA gather.  The sparse items' gather
+            // is dominated by the sub-grid load, so it also runs two work-groups of at
+            // most 256 threads per CU, one loading while the other computes.
             const int dense_min = std::max(1, tl.K / 2);
-            SE_GPM(16, dense_min);
+            Tiling tv = tl;
+            tv.block = tl.mblock;
+            interp_pencil_modes<16>(rec, item_tile, item_beg, item_end, n_items, grid, g, tv, N, phi, force, sys,
+                                    dense_min, s);
             interp_mfma16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
                                 dense_min, s);
         } else {
