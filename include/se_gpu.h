@@ -52,8 +52,9 @@ typedef struct {
     int n_max;         /* particle capacity */
     int tile;          /* tile side in grid cells (at most 16), 0 = automatic */
     int chunk;         /* max particles per work item, 0 = automatic */
-    int block;         /* threads per work-group of the spread and the vector gathers (the P = 16
-                          MFMA gather uses at most 256), 0 = automatic */
+    int block;         /* threads per work-group (<= 512) of every kernel (the P = 16 MFMA gather
+                          uses at most 256); 0 = each kernel's own default: 768 for the FP32
+                          register-ring spread, 256 for the MFMA gather, 512 otherwise */
     int flags;         /* SE_PLAN_* bits */
     double slab;       /* SE2P: particles have 0 <= z < slab; SE1P: particles have
                           0 <= x, y < slab (ignored for SE3P) */
@@ -177,7 +178,8 @@ const char* se_gpu_last_error(void);
  * work-group -- and records the fastest in a process-wide table keyed by (periodicity,
  * M, P, precision, ceil(log2 N)).  Plans created afterwards with tile = chunk = block = 0
  * take the recorded tiling of their geometry at the nearest size band (group plans use
- * particles per system), including the plans the batch executor creates.  out = {tile, chunk, block} and *ms =
+ * particles per system), including the plans the batch executor creates.  out = {tile, chunk, block}
+ * (block 0: the kernels' own defaults, one of the candidates) and *ms =
  * its solve time.  se_tune allocates its own scratch and synchronises `hip_stream`.
  * se_tune_save / se_tune_load write and read the table as text (load returns the number
  * of entries read, or an error code); se_tune_lookup returns 1 and the tiling a plan of

@@ -79,6 +79,7 @@ struct Tiling {
     int pencil;       // 1: register-ring pencil kernels (FP32, P in the compiled set)
     int mfma;         // 1: the pencil gather runs on MFMA (P = 16)
     int mblock;       // threads per work-group of the MFMA gather
+    int sblock;       // threads per work-group of the register-ring (pencil) spread
 };
 
 // Walk over the S^3 points of a tile sub-grid, e = (a S + b) S + c, visited as e = tid,

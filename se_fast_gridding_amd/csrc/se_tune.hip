@@ -128,7 +128,7 @@ int se_tune(const se_plan_desc* desc, int N, int systems, int quantities, void* 
                             !(desc->flags & SE_PLAN_GENERIC_KERNELS);
         std::vector<int> tiles = pencil ? std::vector<int>{4, 8, 12} : std::vector<int>{6, 8, 12, 16};
         const int chunks[] = {512, 1024, 2048, 4096};
-        const int blocks[] = {256, 512};
+        const int blocks[] = {0, 256};   // 0: each kernel's own default (spread 768, gathers 512 / 256)
         // `systems` copies of the system, solved as one group (each with its own outputs)
         std::vector<se_system> sys(systems);
         for (int i = 0; i < systems; ++i) {
@@ -227,7 +227,7 @@ int se_tune_load(const char* path)
         if (line[0] == '#') continue;
         if (std::sscanf(line, "%d %d %d %d %d %d %d %d %f", &per, &M, &P, &fp32, &band, &tile, &chunk, &block, &ms) ==
                 9 &&
-            tile > 0 && chunk > 0 && block > 0) {
+            tile > 0 && chunk > 0 && block >= 0) {
             g_tune[TuneKey{per, M, P, fp32, band}] = TuneVal{tile, chunk, block, ms};
             ++n;
         }

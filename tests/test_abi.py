@@ -149,11 +149,11 @@ def test_tuning_table_save_load_lookup(tmp_path):
     path = tmp_path / "tune.txt"
     path.write_text("# periodicity M P fp32 log2N tile chunk block ms\n"
                     "3 64 16 1 14 8 1024 256 0.5\n"
-                    "3 64 16 1 20 12 4096 512 2.0\n"
+                    "3 64 16 1 20 12 4096 0 2.0\n"
                     "2 48 11 1 15 4 512 512 0.3\n")
     assert lib.se_tune_load(str(path).encode()) == 3
     assert se.tune_lookup(64, 16, 1.0, 20.0, 16000) == (8, 1024, 256)       # band 14
-    assert se.tune_lookup(64, 16, 1.0, 20.0, 900000) == (12, 4096, 512)     # band 20
+    assert se.tune_lookup(64, 16, 1.0, 20.0, 900000) == (12, 4096, 0)       # band 20 (0: kernel defaults)
     assert se.tune_lookup(64, 16, 1.0, 20.0, 60000) == (8, 1024, 256)       # band 16: nearer 14
     assert se.tune_lookup(48, 11, 1.0, 15.0, 30000, periodicity=2, slab=1.0) == (4, 512, 512)
     assert se.tune_lookup(64, 16, 1.0, 20.0, 16000, precision=se.PREC_FP64) is None
