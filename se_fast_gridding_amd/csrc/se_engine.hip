@@ -102,10 +102,10 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     // it runs two waves per SIMD either way, limited by registers.  (The vector gathers
     // measured fastest at 512.)
     tl.mblock = std::min(tl.block, 256);
-    // The pencil spread fills the LDS with one FP64 sub-grid, so one work-group per CU;
-    // at 768 threads (12 waves, 3 per SIMD; its registers fit 168 per lane at that size)
-    // it hides more latency than at 512.  An explicit block applies to every kernel.
-    tl.sblock = block > 0 ? block : 768;
+    // The pencil spread's FP64 sub-grid allows one or two work-groups per CU; at its
+    // bound (768 threads for P = 16, 12 waves; 1024 for P = 8 and 11) it hides more
+    // latency than at 512.  An explicit block applies to every kernel.
+    tl.sblock = block > 0 ? block : spread_pencil_max_threads(P);
     return tl;
 }
 

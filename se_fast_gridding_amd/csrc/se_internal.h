@@ -156,6 +156,10 @@ __host__ __device__ __forceinline__ int count_slot(int k, int nslots)
     return hi + 0x10000 <= nslots ? (hi | (int)(((unsigned)k * 40503u) & 0xFFFFu)) : k;
 }
 
+// Work-group bound of the register-ring spread: the P = 16 kernel fits 168 registers
+// (3 waves per SIMD) at 768 threads; the smaller supports fit 128 at 1024.
+constexpr int spread_pencil_max_threads(int P) { return P == 16 ? 768 : 1024; }
+
 // Particle record written by the bucketing scatter, read by spread and gather.
 template <typename T>
 struct alignas(16) Rec {

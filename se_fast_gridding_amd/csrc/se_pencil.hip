@@ -212,7 +212,7 @@ struct GatherRing<P, P, R, FORCE> {
 }  // namespace
 
 template <int P>
-__global__ __launch_bounds__(768) void k_spread_pencil(const Rec<float>* __restrict__ rec,
+__global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(const Rec<float>* __restrict__ rec,
                                                        const int* __restrict__ item_beg,
                                                        const int* __restrict__ item_end,
                                                        const int* __restrict__ n_items, Geom g, Tiling tl,
@@ -835,6 +835,8 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int*
                           const Geom& g, const Tiling& tl, float* partials, hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(double);
+    SE_REQUIRE(tl.sblock >= 64 && tl.sblock % 64 == 0 && tl.sblock <= spread_pencil_max_threads(g.P), SE_ERR_ARG,
+               "spread work-group size past the kernel's bound");
     dim3 gr(tl.max_items), bl(tl.sblock);
 #define SE_SP(PV)                                                                                            \
     do {                                                                                                     \
