@@ -106,6 +106,7 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     // bound (768 threads for P = 16, 12 waves; 1024 for P = 8 and 11) it hides more
     // latency than at 512.  An explicit block applies to every kernel.
     tl.sblock = block > 0 ? block : spread_pencil_max_threads(P);
+    tl.block_auto = block > 0 ? 0 : 1;
     return tl;
 }
 

@@ -80,6 +80,7 @@ struct Tiling {
     int mfma;         // 1: the pencil gather runs on MFMA (P = 16)
     int mblock;       // threads per work-group of the MFMA gather
     int sblock;       // threads per work-group of the register-ring (pencil) spread
+    int block_auto;   // 1: block was not given, so each kernel may take its own size
 };
 
 // Walk over the S^3 points of a tile sub-grid, e = (a S + b) S + c, visited as e = tid,

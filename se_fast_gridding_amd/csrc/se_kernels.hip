@@ -546,8 +546,13 @@ void launch_merge(const T* partials, const int* chunk_start, const int4* seg, co
 // the derivative factors zf = -c (t - o h) (SE_fgg.c:693-709):
 // F = h^3 q sum G W zf = (q/2) grad phi.
 // ================================================================================
+// The generic gather's work-group bound: 768 threads (3 waves per SIMD in 168 registers)
+// where the FP64 kernel fits them (up to 8 owned pairs per lane, P <= 22), else 512.
+template <typename T, int UM>
+constexpr int interp_tiles_max_threads() { return (sizeof(T) == 8 && UM <= 8) ? 768 : 512; }
+
 template <typename T, int PT, int UM, bool POT, bool FORCE>
-__global__ __launch_bounds__(512) void k_interp_tiles(
+__global__ __launch_bounds__((interp_tiles_max_threads<T, UM>())) void k_interp_tiles(
     const Rec<T>* __restrict__ rec, const int* __restrict__ item_tile, const int* __restrict__ item_beg,
     const int* __restrict__ item_end, const int* __restrict__ n_items, const T* __restrict__ grid, Geom g,
     Tiling tl, int N, double* __restrict__ phi, double* __restrict__ force, const SysDesc* __restrict__ sys)
@@ -660,7 +665,8 @@ static void interp_dispatch_mode(const Rec<T>* rec, const int* item_tile, const 
                                  double* phi, double* force, const SysDesc* sys, hipStream_t s)
 {
     size_t lds = (size_t)tl.sub_elems * sizeof(T);
-    dim3 gr(tl.max_items), bl(tl.block);
+    // at its bound when the block size is automatic: FP64 P = 21 at C2, 3.65 -> 3.13 ms
+    dim3 gr(tl.max_items), bl(tl.block_auto ? interp_tiles_max_threads<T, UM>() : tl.block);
 #define SE_GATHER(POT, FORCE)                                                                                  \
     do {                                                                                                       \
         allow_lds((const void*)k_interp_tiles<T, PT, UM, POT, FORCE>, lds);                                    \
