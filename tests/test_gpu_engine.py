@@ -295,3 +295,17 @@ def test_capacity_is_enforced():
     with pytest.raises(RuntimeError, match="capacity"):
         plan.solve(x, q, torch.zeros(11, dtype=torch.float64, device=DEV))
     plan.close()
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-7, 1e-10])
+def test_selected_parameters_meet_the_tolerance_on_the_gpu(oracle, tol):
+    """se_select_params drives the FP64 engine: a plan built from the (P, M, m) chosen for
+    an rms potential error of tol solves within tol of the oracle's direct k-space sum
+    (|k| <= (M-1)/2, the layers test_accuracy.c:55 uses)."""
+    x, q = system(120, 91)
+    L, xi = 1.0, 6.0
+    P, M, m, est = se.select_params(tol, L, xi, q, potential=True, force=False)
+    print("case select tol=%g P=%d M=%d est=%.2e" % (tol, P, M, est), flush=True)
+    phi, _, _ = solve(x, q, M, P, xi, se.PREC_FP64, force=False, m=m)
+    ref = oracle.direct3p(x, q, (L, L, L), xi, (M - 1) // 2)
+    assert np.sqrt(np.mean((phi - ref) ** 2)) <= tol
