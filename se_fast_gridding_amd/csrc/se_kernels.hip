@@ -449,6 +449,17 @@ __device__ __forceinline__ void merge_add<double>(double* acc, double* cmp, int 
 
 constexpr int MERGE_U = 8;   // partial rows in flight per thread in the merge
 
+// LDS of the merge: the tile's accumulators (and Kahan terms for FP64), rounded up to 16
+// bytes, then the segment and item-range tables (3 sstride int4 + sstride^3 int2)
+__host__ __device__ inline size_t merge_acc_bytes(int TS, size_t esize)
+{
+    return ((size_t)TS * TS * TS * sizeof(double) * (esize == 8 ? 2 : 1) + 15) / 16 * 16;
+}
+inline size_t merge_lds_bytes(int TS, size_t esize, int sstride)
+{
+    return merge_acc_bytes(TS, esize) + (size_t)3 * sstride * sizeof(int4) + (size_t)sstride * sstride * sstride * sizeof(int2);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ partials,
                                                      const int* __restrict__ chunk_start,
@@ -475,27 +486,44 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* acc = reinterpret_cast<double*>(smem_raw);   // [x][y][z] of the tile
     double* cmp = acc + TS * TS * TS;                    // Kahan compensation (FP64 partials)
+    // the tile's source boxes: the segments of each dimension and, per segment triple,
+    // the item range of its source tile, fetched in parallel into LDS once instead of one
+    // dependent global load per triple
+    int4* sA = reinterpret_cast<int4*>(smem_raw + merge_acc_bytes(TS, sizeof(T)));
+    int4* sB = sA + sstride;
+    int4* sC = sB + sstride;
+    int2* sR = reinterpret_cast<int2*>(sC + sstride);
+    const int na = nseg[tx], nb = nseg[ty], nc = nsegz[tz], ntri = na * nb * nc;
     for (int e = threadIdx.x; e < V; e += 256) {
         acc[e] = 0.0;
         if (sizeof(T) == 8) cmp[e] = 0.0;
     }
+    if (threadIdx.x < na) sA[threadIdx.x] = seg[tx * sstride + threadIdx.x];
+    if (threadIdx.x < nb) sB[threadIdx.x] = seg[ty * sstride + threadIdx.x];
+    if (threadIdx.x < nc) sC[threadIdx.x] = segz[tz * sstride + threadIdx.x];
+    __syncthreads();
+    for (int t = threadIdx.x; t < ntri; t += 256) {
+        const int a = t / (nb * nc), b = (t / nc) % nb, c = t % nc;
+        const int src = (sA[a].x * NT + sB[b].x) * NTz + sC[c].x;
+        sR[t] = make_int2(chunk_start[src], chunk_start[src + 1]);
+    }
     __syncthreads();
     const int lz = threadIdx.x & 15, lr = threadIdx.x >> 4;   // z lane, row lane
-    const int na = nseg[tx], nb = nseg[ty], nc = nsegz[tz];
     for (int a = 0; a < na; ++a) {
-        const int4 A = seg[tx * sstride + a];
+        const int4 A = sA[a];
         for (int b = 0; b < nb; ++b) {
-            const int4 B = seg[ty * sstride + b];
+            const int4 B = sB[b];
             for (int c = 0; c < nc; ++c) {
-                const int4 C = segz[tz * sstride + c];
-                const int src = (A.x * NT + B.x) * NTz + C.x;
-                const int i1 = chunk_start[src + 1], rows = A.w * B.w;
+                const int4 C = sC[c];
+                const int2 items = sR[(a * nb + b) * nc + c];
+                if (items.x == items.y) continue;   // uniform: no item, no adds, no barrier
+                const int rows = A.w * B.w;
                 // A point of this segment triple belongs to one thread (z run lz, row lr +
                 // 16 j; runs are at most the tile side, <= 16), for every item of the box,
                 // so the items need no barrier between them; the barrier after the triple
                 // orders the triples.  U loads are issued before their adds.
                 if (lz < C.w) {
-                    for (int it = chunk_start[src]; it < i1; ++it) {
+                    for (int it = items.x; it < items.y; ++it) {
                         const T* base = partials + (size_t)it * tl.part_elems + C.y + lz;
                         for (int r0 = lr; r0 < rows; r0 += 16 * MERGE_U) {
                             T v[MERGE_U];
@@ -532,7 +560,7 @@ void launch_merge(const T* partials, const int* chunk_start, const int4* seg, co
     if (nsys <= 0) return;
     const int cols8 = (tl.NT * tl.NT + 7) / 8;
     dim3 gr(8 * cols8 * tl.NTz, nsys), bl(256);
-    const size_t lds = (size_t)tl.TS * tl.TS * tl.TS * sizeof(double) * (sizeof(T) == 8 ? 2 : 1);
+    const size_t lds = merge_lds_bytes(tl.TS, sizeof(T), sstride);
     allow_lds((const void*)k_merge_tiles<T>, lds);
     hipLaunchKernelGGL(k_merge_tiles<T>, gr, bl, lds, s, partials, chunk_start, seg, nseg, segz, nsegz, sstride, g,
                        tl, grid);
