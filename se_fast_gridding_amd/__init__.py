@@ -228,6 +228,14 @@ def capi():
     if _capi is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libse_fgg_amd.so is not built (%s): run __graft_entry__.build()" % LIB_PATH)
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).  Loading
+        # torch first makes this library bind to that copy, so the process has one HIP
+        # runtime; loaded the other way round, torch's later device initialisation finds
+        # no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIG.items():
             f = getattr(lib, name)

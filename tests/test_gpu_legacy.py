@@ -275,3 +275,59 @@ def test_spectral_ewald_driver_se2p_se1p(oracle, per):
     torch.cuda.synchronize()
     plan.close()
     assert rms_rel(F, Ft.cpu().numpy()) < 1e-13
+
+
+def test_spectral_ewald_driver_fp32_p16(oracle):
+    """spectral_ewald after se_set_precision(SE_FP32) at P = 16 runs the FP32 register-
+    ring spread and the matrix-core gather behind the legacy surface: potential and force
+    within FP32 tolerance of the oracle's FP64 solve of the same (M, P, m, xi)."""
+    lib = se.capi()
+    rng = np.random.default_rng(5)
+    N, P, M, xi = 20000, 16, 32, 8.0   # about 1000 particles per tile: the dense (MFMA) items
+    x = rng.random(3 * N)
+    q = rng.uniform(-1, 1, N)
+    q -= q.mean()
+    opt = se.SE_opt()
+    opt.m = 0.95 * np.sqrt(np.pi * P)
+    opt.box[0] = opt.box[1] = opt.box[2] = 1.0
+    opt.P, opt.M, opt.N = P, M, N
+    F = np.zeros(3 * N)
+    phi = np.zeros(N)
+    print("case driver fp32 N=%d P=%d M=%d" % (N, P, M), flush=True)
+    lib.se_set_precision(1)
+    lib.se_set_quantity(3)
+    try:
+        assert lib.spectral_ewald(dp(x), dp(q), opt, xi, dp(F), dp(phi)) == 0
+    finally:
+        lib.se_set_quantity(1)
+        lib.se_set_precision(0)
+    assert rms_rel(phi, oracle.se3p(x, q, 1.0, M, P, opt.m, xi)) < 1e-5
+    assert rms_rel(F, oracle.se3p(x, q, 1.0, M, P, opt.m, xi, force=True)) < 1e-5
+
+
+def test_legacy_call_before_torch_leaves_torch_a_gpu():
+    """A process that calls the legacy surface before it touches torch must still find the
+    GPU from torch afterwards (torch ships its own HIP runtime under the same soname; the
+    package loads torch first so that one runtime serves both).  Checked in a fresh child
+    process, where nothing has initialised the GPU yet."""
+    import subprocess
+    import sys
+    import os
+    code = (
+        "import numpy as np, se_fast_gridding_amd as se\n"
+        "lib = se.capi()\n"
+        "N, P, M = 50, 8, 16\n"
+        "rng = np.random.default_rng(0)\n"
+        "x = rng.random(3 * N); q = rng.uniform(-1, 1, N); q -= q.mean()\n"
+        "opt = se.SE_opt(); opt.m = 0.95 * np.sqrt(np.pi * P)\n"
+        "opt.box[0] = opt.box[1] = opt.box[2] = 1.0; opt.P, opt.M, opt.N = P, M, N\n"
+        "phi = np.zeros(N)\n"
+        "dp = lambda a: a.ctypes.data_as(__import__('ctypes').POINTER(__import__('ctypes').c_double))\n"
+        "assert lib.spectral_ewald(dp(x), dp(q), opt, 4.0, dp(phi), None) == 0\n"
+        "import torch\n"
+        "assert torch.cuda.is_available() and torch.cuda.device_count() >= 1\n"
+        "t = torch.ones(4, device='cuda'); assert float(t.sum()) == 4.0\n"
+        "print('ok')\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
