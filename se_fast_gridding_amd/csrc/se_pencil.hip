@@ -385,7 +385,7 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
             N = sys[si].N;
         }
         const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
-        load_tile<true>(sub, grid, S, M, Mz, ox, oy, oz, tid, nthr);
+        load_tile_dma<true>(sub, grid, S, M, Mz, ox, oy, oz, tid, nthr);
     }
     __syncthreads();
 
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
             N = sys[si].N;
         }
         const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
-        load_tile<true>(sub, grid, S, M, Mz, ox, oy, oz, tid, nthr);
+        load_tile_dma<true>(sub, grid, S, M, Mz, ox, oy, oz, tid, nthr);
     }
     __syncthreads();
 
@@ -858,7 +858,7 @@ static void interp_mfma16_modes(const Rec<float>* rec, const int* item_tile, con
                                 const Tiling& tl, int N, double* phi, double* force, const SysDesc* sys,
                                 int dense_min, hipStream_t s)
 {
-    const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
+    const size_t lds = ((size_t)tl.S * tl.S * tl.S + LDS_TILE_PAD) * sizeof(float);
     dim3 gr(tl.max_items), bl(tl.mblock);
 #define SE_GM(POT, FORCE)                                                                                     \
     do {                                                                                                      \
@@ -881,7 +881,7 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
                                 const Tiling& tl, int N, double* phi, double* force, const SysDesc* sys,
                                 int dense_min, hipStream_t s)
 {
-    const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(float);
+    const size_t lds = ((size_t)tl.S * tl.S * tl.S + LDS_TILE_PAD) * sizeof(float);
     dim3 gr(tl.max_items), bl(tl.block);
 #define SE_GP(POT, FORCE)                                                                                     \
     do {                                                                                                      \
