@@ -145,28 +145,31 @@ __device__ __forceinline__ void load_tile(T* sub, const T* __restrict__ grid, in
     }
 }
 
-// The same load for FP32 sub-grids by LDS DMA (global_load_lds_dword): wave w takes
-// chunks of 64 consecutive points e0 .. e0 + 63 (e0 = 64 (w + nw k)) and lane l loads
-// the grid value of point e0 + l straight into sub[e0 + l], so a load holds no register
+// The same load by LDS DMA (global_load_lds_dword): wave w takes chunks of 64 dwords of
+// the sub-grid, EPW = 256 / sizeof(T) consecutive points e0 .. e0 + EPW - 1
+// (e0 = EPW (w + nw k)), and lane l loads dword l % DPE of point e0 + l / DPE
+// (DPE = sizeof(T) / 4 dwords per point) straight into LDS, so a load holds no register
 // and every load of the tile can be in flight at once.  In the last chunk the lanes past
-// S^3 read the last point and write into the 64 floats of padding that the caller
-// allocates after the sub-grid (LDS_TILE_PAD).  Returns with the loads complete (vmcnt
-// 0); the caller's barrier then publishes the sub-grid.
+// S^3 read the last point and write into the LDS_TILE_PAD points of padding that the
+// caller allocates after the sub-grid.  Returns with the loads complete (vmcnt 0); the
+// caller's barrier then publishes the sub-grid.
 constexpr int LDS_TILE_PAD = 64;
-template <bool YXZ>
-__device__ __forceinline__ void load_tile_dma(float* sub, const float* __restrict__ grid, int S, int M, int Mz, int ox,
-                                              int oy, int oz, int tid, int nthr)
+template <bool YXZ, typename T>
+__device__ __forceinline__ void load_tile_dma(T* sub, const T* __restrict__ grid, int S, int M, int Mz, int ox, int oy,
+                                              int oz, int tid, int nthr)
 {
+    static_assert(sizeof(T) % 4 == 0, "points are whole dwords");
+    constexpr int DPE = (int)(sizeof(T) / 4), EPW = 64 / DPE;
     const int S3 = S * S * S, lane = tid & 63, nw = nthr >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    SubWalk w(S, 64 * wave + lane, 64 * nw);   // (a, b) = (iy, ix) if YXZ, else (ix, iy); c = iz
-    for (int e0 = 64 * wave; e0 < S3; e0 += 64 * nw) {
-        const bool in = e0 + lane < S3;
+    SubWalk w(S, EPW * wave + lane / DPE, EPW * nw);   // (a, b) = (iy, ix) if YXZ, else (ix, iy); c = iz
+    for (int e0 = EPW * wave; e0 < S3; e0 += EPW * nw) {
+        const bool in = e0 + lane / DPE < S3;
         const int a = in ? w.a : S - 1, b = in ? w.b : S - 1, c = in ? w.c : S - 1;
         const int ix = YXZ ? b : a, iy = YXZ ? a : b;
         const int gx = wrap_index(ox + ix, M), gy = wrap_index(oy + iy, M), gz = wrap_index(oz + c, Mz);
-        __builtin_amdgcn_global_load_lds(grid + ((long)gx * M + gy) * Mz + gz,
-                                         (__attribute__((address_space(3))) void*)(sub + e0), 4, 0, 0);
+        const unsigned* src = reinterpret_cast<const unsigned*>(grid + ((long)gx * M + gy) * Mz + gz) + lane % DPE;
+        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(sub + e0), 4, 0, 0);
         w.next();
     }
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): every DMA write has landed in LDS
