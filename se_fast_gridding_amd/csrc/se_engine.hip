@@ -141,6 +141,9 @@ struct se_plan {
     double L = 1, xi = 1, m = 1, eta = 0;
     double R2p = 0;   // SE2P / SE1P kernel truncation radius
     hipStream_t stream = nullptr;
+    // the gather's second stream (plans whose gather splits items between two kernels)
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int n_cap = 0;
     int last_N = -1;
     // grouped solves: up to s_cap systems of this geometry stacked in one pass
@@ -209,6 +212,9 @@ struct se_plan {
         if (info) rocfft_execution_info_destroy(info);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        if (join_ev) (void)hipEventDestroy(join_ev);
+        if (side) (void)hipStreamDestroy(side);
         for (void* p : allocs) (void)hipFree(p);
     }
     void rec_ev(int i)
@@ -522,10 +528,23 @@ void run_interp(se_plan* pl, double* phi, double* force)
     tlc.max_items = std::min(pl->tl.max_items, pl->last_S * pl->tl.ntiles + pl->last_N / pl->tl.chunk + 1);
     bool done = false;
     if constexpr (sizeof(T) == 4) {
+        // The matrix-core gather and the vector gather of the sparse items run concurrently
+        // (the second on the plan's side stream) only where both have work at scale: one
+        // system of at least 2^20 particles whose mean tile count is below twice the dense
+        // threshold or whose edge tiles (M not a multiple of the tile side) fall below it.
+        // Elsewhere the stream fork and join cost more than the overlap gains.
+        const Tiling& tl = pl->tl;
+        bool split = false;
+        if (tl.mfma && !pl->last_group && pl->last_N >= (1 << 20)) {
+            const double mean = (double)pl->last_N / tl.ntiles, dmin = mfma_dense_min(tl);
+            const int edge = std::min(pl->g.M % tl.TS ? pl->g.M % tl.TS : tl.TS,
+                                      pl->g.Mz % tl.TS ? pl->g.Mz % tl.TS : tl.TS);
+            split = mean >= dmin && (mean < 2 * dmin || mean * edge / tl.TS < dmin);
+        }
         if (pl->tl.pencil)
             done = launch_interp_pencil((const Rec<float>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end,
                                         pl->n_items, (const float*)pl->grid, pl->g, tlc, pl->last_N, phi, force, sys,
-                                        pl->stream);
+                                        pl->stream, split ? pl->side : nullptr, pl->fork_ev, pl->join_ev);
     }
     if (!done)
         launch_interp_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
@@ -749,6 +768,11 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         }
         SE_FFT(rocfft_execution_info_set_stream(pl->info, pl->stream));
         for (auto& e : pl->ev) SE_HIP(hipEventCreate(&e));
+        if (tl.mfma) {
+            SE_HIP(hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking));
+            SE_HIP(hipEventCreateWithFlags(&pl->fork_ev, hipEventDisableTiming));
+            SE_HIP(hipEventCreateWithFlags(&pl->join_ev, hipEventDisableTiming));
+        }
         SE_HIP(hipStreamSynchronize(pl->stream));
     });
     if (rc != SE_OK) {

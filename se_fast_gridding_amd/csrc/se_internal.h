@@ -191,6 +191,10 @@ __host__ __device__ __forceinline__ int count_slot(int k, int nslots)
 // (3 waves per SIMD) at 768 threads; the smaller supports fit 128 at 1024.
 constexpr int spread_pencil_max_threads(int P) { return P == 16 ? 768 : 1024; }
 
+// The matrix-core gather takes the items with at least this many particles (about one
+// per pencil row, which a 4-particle group needs); the vector gather takes the others.
+inline int mfma_dense_min(const Tiling& tl) { return tl.K / 2 > 1 ? tl.K / 2 : 1; }
+
 // Particle record written by the bucketing scatter, read by spread and gather.
 template <typename T>
 struct alignas(16) Rec {
@@ -260,11 +264,14 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int*
                           const int* n_items, const Geom& g, const Tiling& tl, float* partials,
                           hipStream_t s);
 // sys != null: grouped solve, the outputs and N come from sys[tile / ntiles] and the
-// grids are stacked; phi/force then only select the output mode
+// grids are stacked; phi/force then only select the output mode.  side (with the events
+// fork and join) non-null: where the items split between two kernels, the sparse items'
+// kernel runs on side, concurrently with the other on s, and s waits for it.
 bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg,
                           const int* item_end, const int* n_items, const float* grid,
                           const Geom& g, const Tiling& tl, int N, double* phi, double* force,
-                          const SysDesc* sys, hipStream_t s);
+                          const SysDesc* sys, hipStream_t s, hipStream_t side = nullptr,
+                          hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 template <typename T>
 void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const Geom& g,

@@ -900,7 +900,8 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
 
 bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
-                          double* phi, double* force, const SysDesc* sys, hipStream_t s)
+                          double* phi, double* force, const SysDesc* sys, hipStream_t s, hipStream_t side,
+                          hipEvent_t fork, hipEvent_t join)
 {
 #define SE_GPM(PV, DM) \
     interp_pencil_modes<PV>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys, DM, s)
@@ -913,14 +914,24 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
             // fewer particles than half the tile's (pencil, row) keys (sparse tiles) go to
             // the vector gather, the others to the MFMA gather.  The sparse items' gather
             // is dominated by the sub-grid load, so it also runs two work-groups of at
-            // most 256 threads per CU, one loading while the other computes.
-            const int dense_min = std::max(1, tl.K / 2);
+            // most 256 threads per CU, one loading while the other computes.  The two
+            // kernels write disjoint particles; given a side stream they run concurrently,
+            // so each fills the other's tail.
+            const int dense_min = mfma_dense_min(tl);
             Tiling tv = tl;
             tv.block = tl.mblock;
+            hipStream_t sv = s;
+            if (side) {
+                SE_HIP(hipEventRecord(fork, s));
+                SE_HIP(hipStreamWaitEvent(side, fork, 0));
+                sv = side;
+            }
             interp_pencil_modes<16>(rec, item_tile, item_beg, item_end, n_items, grid, g, tv, N, phi, force, sys,
-                                    dense_min, s);
+                                    dense_min, sv);
+            if (side) SE_HIP(hipEventRecord(join, side));
             interp_mfma16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
                                 dense_min, s);
+            if (side) SE_HIP(hipStreamWaitEvent(s, join, 0));
         } else {
             SE_GPM(16, 0);
         }

@@ -91,16 +91,19 @@ def test_hbm_budget_is_enforced():
     b.close()
 
 
-@pytest.mark.parametrize("spec", [(3, 48, 16, 15.0, se.PREC_FP32), (2, 32, 11, 10.0, se.PREC_FP32),
-                                  (3, 32, 16, 10.0, se.PREC_FP64)], ids=["3p_fp32", "2p_fp32", "3p_fp64"])
+# 3p_fp32_split: N = 1.1M on M = 160 (14^3 tiles of side 12, ~400 particles each, 4-wide
+# edge tiles), so the FP32 P = 16 gather runs the matrix-core kernel (full tiles) and the
+# vector kernel (edge tiles) concurrently on two streams, and the capture holds both
+@pytest.mark.parametrize("spec", [(3, 48, 16, 15.0, se.PREC_FP32, 6000), (2, 32, 11, 10.0, se.PREC_FP32, 6000),
+                                  (3, 32, 16, 10.0, se.PREC_FP64, 6000), (3, 160, 16, 50.0, se.PREC_FP32, 1_100_000)],
+                         ids=["3p_fp32", "2p_fp32", "3p_fp64", "3p_fp32_split"])
 def test_graph_replays_match_eager_solves(spec):
     """se_solve_graph: every replay equals the eager solve of the same inputs, with eager
     work and synchronisations between replays, and follows new array contents."""
-    per, M, P, xi, prec = spec
+    per, M, P, xi, prec, N = spec
     # the order of particles inside a sort bucket follows atomic ranks, so two solves of the
     # same inputs agree to rounding: 1e-12 in FP64, FP32 rounding otherwise
     tol = 1e-12 if prec == se.PREC_FP64 else 2e-6
-    N = 6000
     x, q = system(N, 31)
     plan = se.Plan(M, P, 1.0, xi, N, precision=prec, periodicity=per, slab=1.0 if per == 2 else 0.0,
                    stream=torch.cuda.Stream())
