@@ -424,43 +424,38 @@ void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* ite
 // sums, in a fixed order, every partial value that lands on the tile: for each box of
 // partial sub-grid elements that maps onto the tile (the product of one per-dimension
 // segment from each of the three segment tables) and each work item of the box's source
-// tile.  The box is read row by row (16 lanes per z run, 16 runs per pass), so the loads
-// are z-contiguous and no lane tests points outside the box; the sums live in an LDS
-// accumulator per tile point, and a barrier after each (box, item) keeps the per-point
-// order fixed, so the result does not depend on scheduling.  FP32 partials are summed in
-// FP64; FP64 partials with Kahan compensation.
+// tile.  Each thread owns fixed tile points (point e = thread + 256 j, z fastest) and
+// keeps their sums in registers, so the (box, item) pairs need no barrier between them:
+// the thread walks the pairs in order, MB pairs at a time with all their loads in
+// flight, and adds each value to its point in that order, so the result does not
+// depend on scheduling.  FP32 partials are summed in FP64; FP64 partials with Kahan
+// compensation.
 // seg[t * sstride + a] = (source tile coordinate, local start, output start, length).
 // ================================================================================
 template <typename T>
-__device__ __forceinline__ void merge_add(double* acc, double* cmp, int o, T v);
+__device__ __forceinline__ void merge_add(double& acc, double& cmp, T v);
 template <>
-__device__ __forceinline__ void merge_add<float>(double* acc, double*, int o, float v)
+__device__ __forceinline__ void merge_add<float>(double& acc, double&, float v)
 {
-    acc[o] += (double)v;
+    acc += (double)v;
 }
 template <>
-__device__ __forceinline__ void merge_add<double>(double* acc, double* cmp, int o, double v)
+__device__ __forceinline__ void merge_add<double>(double& acc, double& cmp, double v)
 {
-    const double y = v - cmp[o];
-    const double t = acc[o] + y;
-    cmp[o] = (t - acc[o]) - y;
-    acc[o] = t;
+    const double y = v - cmp;
+    const double t = acc + y;
+    cmp = (t - acc) - y;
+    acc = t;
 }
 
-constexpr int MERGE_U = 8;   // partial rows in flight per thread in the merge
-
-// LDS of the merge: the tile's accumulators (and Kahan terms for FP64), rounded up to 16
-// bytes, then the segment and item-range tables (3 sstride int4 + sstride^3 int2)
-__host__ __device__ inline size_t merge_acc_bytes(int TS, size_t esize)
+// LDS of the merge: the segment and item-range tables (3 sstride int4 + sstride^3 int2)
+inline size_t merge_lds_bytes(int sstride)
 {
-    return ((size_t)TS * TS * TS * sizeof(double) * (esize == 8 ? 2 : 1) + 15) / 16 * 16;
-}
-inline size_t merge_lds_bytes(int TS, size_t esize, int sstride)
-{
-    return merge_acc_bytes(TS, esize) + (size_t)3 * sstride * sizeof(int4) + (size_t)sstride * sstride * sstride * sizeof(int2);
+    return (size_t)3 * sstride * sizeof(int4) + (size_t)sstride * sstride * sstride * sizeof(int2);
 }
 
-template <typename T>
+// JM tile points per thread (TS^3 <= 256 JM); MB (box, item) pairs in flight
+template <typename T, int JM, int MB>
 __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ partials,
                                                      const int* __restrict__ chunk_start,
                                                      const int4* __restrict__ seg, const int* __restrict__ nseg,
@@ -483,21 +478,15 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
     const int ox = tx * TS, oy = ty * TS, oz = tz * TS;
     const int nx = min(TS, M - ox), ny = min(TS, M - oy), nz = min(TS, Mz - oz);
     const int V = nx * ny * nz;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    double* acc = reinterpret_cast<double*>(smem_raw);   // [x][y][z] of the tile
-    double* cmp = acc + TS * TS * TS;                    // Kahan compensation (FP64 partials)
     // the tile's source boxes: the segments of each dimension and, per segment triple,
     // the item range of its source tile, fetched in parallel into LDS once instead of one
     // dependent global load per triple
-    int4* sA = reinterpret_cast<int4*>(smem_raw + merge_acc_bytes(TS, sizeof(T)));
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int4* sA = reinterpret_cast<int4*>(smem_raw);
     int4* sB = sA + sstride;
     int4* sC = sB + sstride;
     int2* sR = reinterpret_cast<int2*>(sC + sstride);
     const int na = nseg[tx], nb = nseg[ty], nc = nsegz[tz], ntri = na * nb * nc;
-    for (int e = threadIdx.x; e < V; e += 256) {
-        acc[e] = 0.0;
-        if (sizeof(T) == 8) cmp[e] = 0.0;
-    }
     if (threadIdx.x < na) sA[threadIdx.x] = seg[tx * sstride + threadIdx.x];
     if (threadIdx.x < nb) sB[threadIdx.x] = seg[ty * sstride + threadIdx.x];
     if (threadIdx.x < nc) sC[threadIdx.x] = segz[tz * sstride + threadIdx.x];
@@ -508,49 +497,78 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
         sR[t] = make_int2(chunk_start[src], chunk_start[src + 1]);
     }
     __syncthreads();
-    const int lz = threadIdx.x & 15, lr = threadIdx.x >> 4;   // z lane, row lane
-    for (int a = 0; a < na; ++a) {
-        const int4 A = sA[a];
-        for (int b = 0; b < nb; ++b) {
-            const int4 B = sB[b];
-            for (int c = 0; c < nc; ++c) {
-                const int4 C = sC[c];
-                const int2 items = sR[(a * nb + b) * nc + c];
-                if (items.x == items.y) continue;   // uniform: no item, no adds, no barrier
-                const int rows = A.w * B.w;
-                // A point of this segment triple belongs to one thread (z run lz, row lr +
-                // 16 j; runs are at most the tile side, <= 16), for every item of the box,
-                // so the items need no barrier between them; the barrier after the triple
-                // orders the triples.  U loads are issued before their adds.
-                if (lz < C.w) {
-                    for (int it = items.x; it < items.y; ++it) {
-                        const T* base = partials + (size_t)it * tl.part_elems + C.y + lz;
-                        for (int r0 = lr; r0 < rows; r0 += 16 * MERGE_U) {
-                            T v[MERGE_U];
-                            int o[MERGE_U];
+    // owned points, packed (x << 20 | y << 10 | z); -1 past the tile
+    int pt[JM];
+    double acc[JM], cmp[JM];
 #pragma unroll
-                            for (int u = 0; u < MERGE_U; ++u) {
-                                const int rr = r0 + 16 * u;
-                                if (rr < rows) {
-                                    const int dx = rr / B.w, dy = rr - dx * B.w;
-                                    v[u] = base[((A.y + dx) * S + (B.y + dy)) * S];
-                                    o[u] = ((A.z + dx) * ny + (B.z + dy)) * nz + C.z + lz;
-                                }
-                            }
+    for (int j = 0; j < JM; ++j) {
+        const int e = threadIdx.x + 256 * j;
+        const int ez = e % nz, ey = (e / nz) % ny, ex = e / (nz * ny);
+        pt[j] = e < V ? (ex << 20) | (ey << 10) | ez : -1;
+        acc[j] = 0.0;
+        cmp[j] = 0.0;
+    }
+    // cursor over the (box, item) pairs in order: box t, item it of [it, ie)
+    int t = 0, it = 0, ie = 0;
+    while (t < ntri && sR[t].x == sR[t].y) ++t;
+    if (t < ntri) {
+        it = sR[t].x;
+        ie = sR[t].y;
+    }
+    while (t < ntri) {
+        T v[MB][JM];
+        unsigned msk[MB];
 #pragma unroll
-                            for (int u = 0; u < MERGE_U; ++u)
-                                if (r0 + 16 * u < rows) merge_add<T>(acc, cmp, o[u], v[u]);
-                        }
+        for (int m = 0; m < MB; ++m) {
+            msk[m] = 0;
+            if (t < ntri) {
+                const int a = t / (nb * nc), b = (t / nc) % nb, c = t % nc;
+                const int4 A = sA[a], B = sB[b], C = sC[c];
+                const T* base = partials + (size_t)it * tl.part_elems;
+#pragma unroll
+                for (int j = 0; j < JM; ++j) {
+                    const int px = pt[j] >> 20, py = (pt[j] >> 10) & 1023, pz = pt[j] & 1023;
+                    const int dx = px - A.z, dy = py - B.z, dz = pz - C.z;
+                    const bool in = pt[j] >= 0 && (unsigned)dx < (unsigned)A.w && (unsigned)dy < (unsigned)B.w &&
+                                    (unsigned)dz < (unsigned)C.w;
+                    v[m][j] = in ? base[((A.y + dx) * S + (B.y + dy)) * S + C.y + dz] : (T)0;
+                    msk[m] |= (unsigned)in << j;
+                }
+                if (++it == ie) {
+                    ++t;
+                    while (t < ntri && sR[t].x == sR[t].y) ++t;
+                    if (t < ntri) {
+                        it = sR[t].x;
+                        ie = sR[t].y;
                     }
                 }
-                __syncthreads();
             }
         }
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int j = 0; j < JM; ++j)
+                if (msk[m] >> j & 1u) merge_add<T>(acc[j], cmp[j], v[m][j]);
     }
-    for (int e = threadIdx.x; e < V; e += 256) {
-        const int ez = e % nz, ey = (e / nz) % ny, ex = e / (nz * ny);
-        grid[((long)(ox + ex) * M + (oy + ey)) * Mz + (oz + ez)] = (T)acc[e];
-    }
+#pragma unroll
+    for (int j = 0; j < JM; ++j)
+        if (pt[j] >= 0)
+            grid[((long)(ox + (pt[j] >> 20)) * M + (oy + ((pt[j] >> 10) & 1023))) * Mz + (oz + (pt[j] & 1023))] =
+                (T)acc[j];
+}
+
+template <typename T, int JM>
+static void launch_merge_jm(const T* partials, const int* chunk_start, const int4* seg, const int* nseg,
+                            const int4* segz, const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid,
+                            int nsys, hipStream_t s)
+{
+    constexpr int MB = sizeof(T) == 4 ? 4 : 2;
+    const int cols8 = (tl.NT * tl.NT + 7) / 8;
+    dim3 gr(8 * cols8 * tl.NTz, nsys), bl(256);
+    const size_t lds = merge_lds_bytes(sstride);
+    allow_lds((const void*)k_merge_tiles<T, JM, MB>, lds);
+    hipLaunchKernelGGL((k_merge_tiles<T, JM, MB>), gr, bl, lds, s, partials, chunk_start, seg, nseg, segz, nsegz,
+                       sstride, g, tl, grid);
 }
 
 template <typename T>
@@ -558,12 +576,16 @@ void launch_merge(const T* partials, const int* chunk_start, const int4* seg, co
                   const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid, int nsys, hipStream_t s)
 {
     if (nsys <= 0) return;
-    const int cols8 = (tl.NT * tl.NT + 7) / 8;
-    dim3 gr(8 * cols8 * tl.NTz, nsys), bl(256);
-    const size_t lds = merge_lds_bytes(tl.TS, sizeof(T), sstride);
-    allow_lds((const void*)k_merge_tiles<T>, lds);
-    hipLaunchKernelGGL(k_merge_tiles<T>, gr, bl, lds, s, partials, chunk_start, seg, nseg, segz, nsegz, sstride, g,
-                       tl, grid);
+    const int pts = tl.TS * tl.TS * tl.TS;
+    SE_REQUIRE(tl.TS <= 16, SE_ERR_ARG, "merge: tile side above 16");
+    if (pts <= 256 * 2)
+        launch_merge_jm<T, 2>(partials, chunk_start, seg, nseg, segz, nsegz, sstride, g, tl, grid, nsys, s);
+    else if (pts <= 256 * 7)
+        launch_merge_jm<T, 7>(partials, chunk_start, seg, nseg, segz, nsegz, sstride, g, tl, grid, nsys, s);
+    else if (pts <= 256 * 11)
+        launch_merge_jm<T, 11>(partials, chunk_start, seg, nseg, segz, nsegz, sstride, g, tl, grid, nsys, s);
+    else
+        launch_merge_jm<T, 16>(partials, chunk_start, seg, nseg, segz, nsegz, sstride, g, tl, grid, nsys, s);
 }
 
 // ================================================================================
