@@ -426,9 +426,9 @@ void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* ite
 // segment from each of the three segment tables) and each work item of the box's source
 // tile.  Each thread owns fixed tile points (point e = thread + 256 j, z fastest) and
 // keeps their sums in registers, so the (box, item) pairs need no barrier between them:
-// the thread walks the pairs in order, MB pairs at a time with all their loads in
-// flight, and adds each value to its point in that order, so the result does not
-// depend on scheduling.  FP32 partials are summed in FP64; FP64 partials with Kahan
+// the thread walks the pairs in order (MB at a time with their loads in flight; one
+// measured fastest, as more registers per thread cost occupancy), and adds each value to
+// its point in that order, so the result does not depend on scheduling.  FP32 partials are summed in FP64; FP64 partials with Kahan
 // compensation.
 // seg[t * sstride + a] = (source tile coordinate, local start, output start, length).
 // ================================================================================
@@ -562,7 +562,7 @@ static void launch_merge_jm(const T* partials, const int* chunk_start, const int
                             const int4* segz, const int* nsegz, int sstride, const Geom& g, const Tiling& tl, T* grid,
                             int nsys, hipStream_t s)
 {
-    constexpr int MB = sizeof(T) == 4 ? 4 : 2;
+    constexpr int MB = 1;   // 1, 2, 3, 4, 8 pairs: 0.061, 0.062, 0.064, 0.066, 0.069 ms (N = 1M, M = 128)
     const int cols8 = (tl.NT * tl.NT + 7) / 8;
     dim3 gr(8 * cols8 * tl.NTz, nsys), bl(256);
     const size_t lds = merge_lds_bytes(sstride);
