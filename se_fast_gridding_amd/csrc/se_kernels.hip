@@ -454,9 +454,11 @@ inline size_t merge_lds_bytes(int sstride)
     return (size_t)3 * sstride * sizeof(int4) + (size_t)sstride * sstride * sstride * sizeof(int2);
 }
 
-// JM tile points per thread (TS^3 <= 256 JM); MB (box, item) pairs in flight
+constexpr int MERGE_THREADS = 256;   // threads per work-group (one tile)
+
+// JM tile points per thread (TS^3 <= MERGE_THREADS JM); MB (box, item) pairs in flight
 template <typename T, int JM, int MB>
-__global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ partials,
+__global__ __launch_bounds__(MERGE_THREADS) void k_merge_tiles(const T* __restrict__ partials,
                                                      const int* __restrict__ chunk_start,
                                                      const int4* __restrict__ seg, const int* __restrict__ nseg,
                                                      const int4* __restrict__ segz, const int* __restrict__ nsegz,
@@ -491,7 +493,7 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
     if (threadIdx.x < nb) sB[threadIdx.x] = seg[ty * sstride + threadIdx.x];
     if (threadIdx.x < nc) sC[threadIdx.x] = segz[tz * sstride + threadIdx.x];
     __syncthreads();
-    for (int t = threadIdx.x; t < ntri; t += 256) {
+    for (int t = threadIdx.x; t < ntri; t += MERGE_THREADS) {
         const int a = t / (nb * nc), b = (t / nc) % nb, c = t % nc;
         const int src = (sA[a].x * NT + sB[b].x) * NTz + sC[c].x;
         sR[t] = make_int2(chunk_start[src], chunk_start[src + 1]);
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const T* __restrict__ parti
     double acc[JM], cmp[JM];
 #pragma unroll
     for (int j = 0; j < JM; ++j) {
-        const int e = threadIdx.x + 256 * j;
+        const int e = threadIdx.x + MERGE_THREADS * j;
         const int ez = e % nz, ey = (e / nz) % ny, ex = e / (nz * ny);
         pt[j] = e < V ? (ex << 20) | (ey << 10) | ez : -1;
         acc[j] = 0.0;
@@ -564,7 +566,7 @@ static void launch_merge_jm(const T* partials, const int* chunk_start, const int
 {
     constexpr int MB = 1;   // 1, 2, 3, 4, 8 pairs: 0.061, 0.062, 0.064, 0.066, 0.069 ms (N = 1M, M = 128)
     const int cols8 = (tl.NT * tl.NT + 7) / 8;
-    dim3 gr(8 * cols8 * tl.NTz, nsys), bl(256);
+    dim3 gr(8 * cols8 * tl.NTz, nsys), bl(MERGE_THREADS);
     const size_t lds = merge_lds_bytes(sstride);
     allow_lds((const void*)k_merge_tiles<T, JM, MB>, lds);
     hipLaunchKernelGGL((k_merge_tiles<T, JM, MB>), gr, bl, lds, s, partials, chunk_start, seg, nseg, segz, nsegz,
@@ -578,11 +580,11 @@ void launch_merge(const T* partials, const int* chunk_start, const int4* seg, co
     if (nsys <= 0) return;
     const int pts = tl.TS * tl.TS * tl.TS;
     SE_REQUIRE(tl.TS <= 16, SE_ERR_ARG, "merge: tile side above 16");
-    if (pts <= 256 * 2)
+    if (pts <= MERGE_THREADS * 2)
         launch_merge_jm<T, 2>(partials, chunk_start, seg, nseg, segz, nsegz, sstride, g, tl, grid, nsys, s);
-    else if (pts <= 256 * 7)
+    else if (pts <= MERGE_THREADS * 7)
         launch_merge_jm<T, 7>(partials, chunk_start, seg, nseg, segz, nsegz, sstride, g, tl, grid, nsys, s);
-    else if (pts <= 256 * 11)
+    else if (pts <= MERGE_THREADS * 11)
         launch_merge_jm<T, 11>(partials, chunk_start, seg, nseg, segz, nsegz, sstride, g, tl, grid, nsys, s);
     else
         launch_merge_jm<T, 16>(partials, chunk_start, seg, nseg, segz, nsegz, sstride, g, tl, grid, nsys, s);
