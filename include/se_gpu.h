@@ -5,7 +5,10 @@
  * interface the fast path runs behind.  A plan owns one grid geometry, its
  * rocFFT plans and all scratch in HBM; particle arrays are DEVICE pointers in the
  * reference's layout (x SoA: x[0..N) y[N..2N) z[2N..3N); q[N]; phi[N]; F SoA[3N])
- * and every call is asynchronous on the plan's stream.  Functions return 0 or a
+ * and every call is asynchronous on the plan's stream (an FP32 P = 16 plan also owns
+ * one internal stream, forked from and joined back to the plan's stream inside the
+ * gather, so the call stays ordered on the plan's stream and can be graph-captured).
+ * Functions return 0 or a
  * negative SE_ERR_* code; se_gpu_last_error() returns the last message.
  *
  * The solve is the reference driver (spectral_ewald.c:83-165) re-staged for the GPU:
