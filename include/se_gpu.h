@@ -255,10 +255,11 @@ int se_real_solve(se_real_plan* plan, int N, const double* d_x, const double* d_
  * stay within `hbm_budget` bytes (0 = 64 GiB); least recently used plans of idle slots
  * are freed to make room, and a job whose plan cannot fit fails with SE_ERR_CAPACITY.
  * Jobs on one slot run in order, jobs on different slots overlap.  A job's result does
- * not depend on its slot or its neighbours beyond rounding: the only run-to-run freedom
- * is the order of particles inside one sort bucket (atomic ranks) and of LDS atomics
- * inside one work item, so repeated solves agree to about 1e-16 (FP64 plans) or to FP32
- * rounding (FP32 plans).
+ * not depend on its slot or its neighbours: particles inside a sort key are ordered by
+ * index, and FP32 plans add integer-valued FP64 values in their LDS sub-grids (exact in
+ * any order), so FP32 results repeat bit for bit, alone and in any mix (keys of more than
+ * 65536 particles, over 16k per grid cell, keep the order of the counting atomics).  FP64
+ * plans add FP64 products with LDS atomics and repeat to about 1e-16.
  * The executor is not thread-safe; the caller serialises calls.
  */
 #define SE_BUSY 1

@@ -259,6 +259,37 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def _check(t, n, name, where):
+    """Refuse, before any C call, an array the kernels would misread: the engine reads and
+    writes raw device pointers of float64 SoA arrays of a fixed length, so a CPU tensor,
+    another dtype, a strided view or a wrong length would fault or be silently misread."""
+    if t is None:
+        return
+    if not hasattr(t, "data_ptr") or not hasattr(t, "is_cuda"):
+        raise ValueError("%s: %s must be a torch tensor, got %s" % (where, name, type(t).__name__))
+    if not t.is_cuda:
+        raise ValueError("%s: %s must be a CUDA (HIP device) tensor" % (where, name))
+    import torch
+    if t.dtype != torch.float64:
+        raise ValueError("%s: %s must be float64, got %s" % (where, name, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s: %s must be contiguous" % (where, name))
+    if t.numel() != n:
+        raise ValueError("%s: %s must hold %d values, got %d" % (where, name, n, t.numel()))
+
+
+def _check_particles(where, x, q, phi=None, force=None):
+    """Checks the SoA particle arrays of one system (x 3N, q N, phi N, force 3N) and returns N."""
+    if q is None or not hasattr(q, "numel"):
+        raise ValueError("%s: q must be a torch tensor" % where)
+    N = q.numel()
+    _check(q, N, "q", where)
+    _check(x, 3 * N, "x", where)
+    _check(phi, N, "phi", where)
+    _check(force, 3 * N, "force", where)
+    return N
+
+
 class Plan:
     """Device-resident SE3P / SE2P / SE1P solver (include/se_gpu.h) driven with torch CUDA tensors.
 
@@ -304,30 +335,40 @@ class Plan:
     __del__ = close
 
     def solve(self, x, q, phi=None, force=None):
-        N = q.numel()
+        N = _check_particles("Plan.solve", x, q, phi, force)
         check(capi().se_solve(self.h, N, _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_solve")
+        self._last_N = N
 
     def solve_group(self, systems):
         """Grouped solve of [(x, q, phi, force), ...] (phi / force may be None, the same
         for every system) in one pass (se_solve_group)."""
         arr = (se_system * len(systems))()
         for i, (x, q, phi, force) in enumerate(systems):
-            arr[i] = se_system(q.numel(), x.data_ptr(), q.data_ptr(), phi.data_ptr() if phi is not None else None,
+            N = _check_particles("Plan.solve_group (system %d)" % i, x, q, phi, force)
+            arr[i] = se_system(N, x.data_ptr(), q.data_ptr(), phi.data_ptr() if phi is not None else None,
                                force.data_ptr() if force is not None else None)
         check(capi().se_solve_group(self.h, len(systems), arr), "se_solve_group")
+        self._last_N = None
 
     def solve_graph(self, x, q, phi=None, force=None):
         """se_solve through a captured HIP graph (captured on the first call with these tensors)."""
-        N = q.numel()
+        N = _check_particles("Plan.solve_graph", x, q, phi, force)
         check(capi().se_solve_graph(self.h, N, _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_solve_graph")
+        self._last_N = N
 
     def spread(self, x, q):
-        check(capi().se_spread(self.h, q.numel(), _ptr(x), _ptr(q)), "se_spread")
+        N = _check_particles("Plan.spread", x, q)
+        check(capi().se_spread(self.h, N, _ptr(x), _ptr(q)), "se_spread")
+        self._last_N = N
 
     def kspace(self):
         check(capi().se_kspace(self.h), "se_kspace")
 
     def interp(self, phi=None, force=None):
+        N = getattr(self, "_last_N", None)
+        if N is not None:
+            _check(phi, N, "phi", "Plan.interp")
+            _check(force, 3 * N, "force", "Plan.interp")
         check(capi().se_interp(self.h, _ptr(phi), _ptr(force)), "se_interp")
 
     def _grid_ptr(self):
@@ -439,7 +480,8 @@ class RealSpace:
     __del__ = close
 
     def solve(self, x, q, phi=None, force=None):
-        check(capi().se_real_solve(self.h, q.numel(), _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_real_solve")
+        N = _check_particles("RealSpace.solve", x, q, phi, force)
+        check(capi().se_real_solve(self.h, N, _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_real_solve")
 
 
 _hip = None
@@ -477,8 +519,9 @@ class Batch:
 
     @staticmethod
     def job(M, P, box, xi, x, q, phi=None, force=None, precision=PREC_FP32, m=0.0, periodicity=3, slab=0.0):
+        N = _check_particles("Batch.job", x, q, phi, force)
         sh = se_job_shape(periodicity, M, P, float(box), float(xi), float(m), float(slab), precision)
-        return se_job(sh, q.numel(), x.data_ptr(), q.data_ptr(), phi.data_ptr() if phi is not None else None,
+        return se_job(sh, N, x.data_ptr(), q.data_ptr(), phi.data_ptr() if phi is not None else None,
                       force.data_ptr() if force is not None else None)
 
     def submit(self, job):

@@ -107,6 +107,10 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     // latency than at 512.  An explicit block applies to every kernel.
     tl.sblock = block > 0 ? block : spread_pencil_max_threads(P);
     tl.block_auto = block > 0 ? 0 : 1;
+    // every launch shape must lie within its kernel's __launch_bounds__: refused here, at
+    // plan creation, rather than by the launch guards at solve time
+    SE_REQUIRE(tl.sblock <= spread_pencil_max_threads(P) && tl.mblock <= 512 && tl.block <= 512, SE_ERR_ARG,
+               "work-group size past a kernel's launch bound");
     return tl;
 }
 
@@ -130,6 +134,8 @@ static hipError_t scan_counts(void* tmp, size_t& bytes, const int* count, int* o
         hipcub::CountingInputIterator<int>(0), SlotCount{count, n});
     return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, it, out, n, s);
 }
+
+
 
 }  // namespace se
 
@@ -163,7 +169,9 @@ struct se_plan {
     std::vector<GroupFft> gffts;   // batched transforms, one pair per group size
     unsigned long gfft_clock = 0;
     // device buffers
-    int *key = nullptr, *rank = nullptr, *count = nullptr, *row_start = nullptr, *tile_count = nullptr,
+    // key / rank: sort key and provisional rank (counter atomics) per particle; memb: the
+    // members of each key; pos: each particle's position in (key, index) order
+    int *key = nullptr, *rank = nullptr, *memb = nullptr, *pos = nullptr, *count = nullptr, *row_start = nullptr, *tile_count = nullptr,
         *nchunk = nullptr, *chunk_start = nullptr, *item_tile = nullptr, *item_beg = nullptr,
         *item_end = nullptr, *n_items = nullptr, *nseg = nullptr, *nsegz = nullptr;
     int4 *seg = nullptr, *segz = nullptr;
@@ -182,6 +190,10 @@ struct se_plan {
         const void *x, *q, *phi, *force;
         hipGraphExec_t exec;
         unsigned long last_use;
+        // state the captured solve leaves behind, restored on every replay so that a later
+        // se_interp / se_plan_info sees this solve and not the last eager call
+        int last_N, last_S;
+        bool last_group;
     };
     std::vector<GraphEntry> graphs;
     unsigned long graph_clock = 0;
@@ -442,7 +454,7 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
                               !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA));
     const size_t M = g.M, Mz = g.Mz, N = d->n_max, ns = n_sys;
     size_t b = 0;
-    b += 2 * N * 4;                                       // key, rank
+    b += 4 * N * 4;                                       // key, rank, members, positions
     b += 2 * (size_t)(ns * tl.nbins + 1) * 4;             // count, row_start
     b += 4 * (size_t)(ns * tl.ntiles + 2) * 4;            // tile_count, nchunk, chunk_start
     b += 3 * (size_t)tl.max_items * 4;                    // items
@@ -489,11 +501,13 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int
     SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->nchunk, pl->chunk_start, nt + 1, s));
     launch_items(pl->tile_count, pl->row_start, pl->chunk_start, nt, tl.K, tl.chunk, pl->item_tile, pl->item_beg,
                  pl->item_end, pl->n_items, s);
+    // position of every particle in (key, index) order: the particle order inside a key
+    // does not depend on the order of the counter atomics
+    launch_stable_positions(pl->key, pl->rank, pl->row_start, N, (int)nb, pl->memb, pl->pos, s);
     if (grp)
-        launch_scatter_group<T>(pl->d_sys, S, blocks, pl->g, tl, pl->key, pl->rank, pl->row_start, (Rec<T>*)pl->rec,
-                                s);
+        launch_scatter_group<T>(pl->d_sys, S, blocks, pl->g, tl, pl->pos, (Rec<T>*)pl->rec, s);
     else
-        launch_scatter<T>(x, q, N, pl->g, tl, pl->key, pl->rank, pl->row_start, (Rec<T>*)pl->rec, s);
+        launch_scatter<T>(x, q, N, pl->g, tl, pl->pos, (Rec<T>*)pl->rec, s);
     pl->rec_ev(1);
     // work items: at most one partial item per non-empty tile plus one per full chunk
     Tiling tlc = tl;
@@ -714,6 +728,8 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
                    "too many systems for the sort keys of this geometry");
         pl->key = (int*)pl->alloc(N * 4);
         pl->rank = (int*)pl->alloc(N * 4);
+        pl->memb = (int*)pl->alloc(N * 4);
+        pl->pos = (int*)pl->alloc(N * 4);
         pl->count = (int*)pl->alloc((ns * tl.nbins + 1) * 4);
         pl->row_start = (int*)pl->alloc((ns * tl.nbins + 1) * 4);
         pl->tile_count = (int*)pl->alloc((ns * tl.ntiles + 1) * 4);
@@ -730,6 +746,13 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         pl->gtab = (double*)pl->alloc((pl->g.per == 3 ? 2 * M + 2 * Mz + 1 : M * M * (Mz / 2 + 1)) * sizeof(double));
         SE_HIP(scan_counts(nullptr, pl->cub_bytes, pl->count, pl->row_start, (int)(ns * tl.nbins + 1),
                            pl->stream));
+        {
+            // one temporary area for the scans of the key counts and of the tile chunks
+            size_t cb = 0;
+            SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, cb, pl->nchunk, pl->chunk_start,
+                                                    (int)(ns * tl.ntiles + 1), pl->stream));
+            pl->cub_bytes = std::max(pl->cub_bytes, cb);
+        }
         if (ns > 1) {
             pl->d_sys = (SysDesc*)pl->alloc(ns * sizeof(SysDesc));
             for (int i = 0; i < 4; ++i) {
@@ -926,6 +949,9 @@ int se_solve_graph(se_plan* pl, int N, const double* x, const double* q, double*
     for (auto& e : pl->graphs)
         if (e.N == N && e.x == x && e.q == q && e.phi == phi && e.force == force) {
             e.last_use = ++pl->graph_clock;
+            pl->last_N = e.last_N;
+            pl->last_S = e.last_S;
+            pl->last_group = e.last_group;
             return guarded([&] { SE_HIP(hipGraphLaunch(e.exec, pl->stream)); });
         }
     // validate eagerly (argument errors must not happen inside a capture)
@@ -958,7 +984,7 @@ int se_solve_graph(se_plan* pl, int N, const double* x, const double* q, double*
             (void)hipGraphExecDestroy(lru->exec);
             pl->graphs.erase(lru);
         }
-        pl->graphs.push_back({N, x, q, phi, force, exec, ++pl->graph_clock});
+        pl->graphs.push_back({N, x, q, phi, force, exec, ++pl->graph_clock, pl->last_N, pl->last_S, pl->last_group});
         SE_HIP(hipGraphLaunch(exec, pl->stream));
     });
     if (graph) (void)hipGraphDestroy(graph);

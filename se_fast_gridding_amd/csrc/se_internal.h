@@ -187,6 +187,47 @@ __host__ __device__ __forceinline__ int count_slot(int k, int nslots)
     return hi + 0x10000 <= nslots ? (hi | (int)(((unsigned)k * 40503u) & 0xFFFFu)) : k;
 }
 
+// Deterministic FP32 spread sums.  The FP32 spreads add FP32 values into an FP64 LDS
+// sub-grid with atomics, whose order varies from run to run.  Each value is scaled by 2^s
+// and rounded to an integer before it is added, with s chosen per work item so that every
+// partial sum of the item is an integer below 2^52 in magnitude: FP64 adds such integers
+// exactly, so the sums -- and the FP32 partials made from them -- do not depend on the
+// order of the atomics.  The bound of a partial sum is sum |q| d <= n max|q| d (each
+// Gaussian factor is at most 1), so s = 50 - ilogb(n max|q| d) keeps it below 2^51; the
+// quantum 2^-s is about n max|q| d 2^-51, far below FP32 resolution.  s is capped so that
+// d 2^s stays a finite float (q d 2^s is then formed without overflow).
+__device__ __forceinline__ int exponent_of(double v)   // ilogb for normal positive v
+{
+    return (int)((__double_as_longlong(v) >> 52) & 0x7ff) - 1023;
+}
+__device__ __forceinline__ int flush_scale(double bound, double d)
+{
+    if (!(bound >= 0x1p-1000) || !(bound < 1e300)) return 0;
+    const int s = 50 - exponent_of(bound);
+    const int smax = 125 - exponent_of(d);
+    return s < smax ? s : smax;
+}
+
+// max |q| over the records [beg, end) of a work item: every thread reduces a strided share,
+// the wave maximum goes to wmax[wave]; after the caller's barrier, item_qmax_read combines
+// them.
+template <typename R>
+__device__ __forceinline__ void item_qmax_part(const R* __restrict__ rec, int beg, int end, int tid, int nthr,
+                                               float* wmax)
+{
+    float m = 0.f;
+    for (int i = beg + tid; i < end; i += nthr) m = fmaxf(m, fabsf((float)rec[i].q));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((tid & 63) == 0) wmax[tid >> 6] = m;
+}
+__device__ __forceinline__ float item_qmax_read(const float* wmax, int nw)
+{
+    float m = 0.f;
+    for (int w = 0; w < nw; ++w) m = fmaxf(m, wmax[w]);
+    return m;
+}
+
 // Work-group bound of the register-ring spread: the P = 16 kernel fits 168 registers
 // (3 waves per SIMD) at 768 threads; the smaller supports fit 128 at 1024.
 constexpr int spread_pencil_max_threads(int P) { return P == 16 ? 768 : 1024; }
@@ -239,9 +280,29 @@ bool tuned_tiling(const se_plan_desc* d, long n_per_system, int& tile, int& chun
 void launch_fill_i32(int* p, long n, int v, hipStream_t s);
 void launch_fill_bytes0(void* p, size_t bytes, hipStream_t s);
 
+// ---- stable sort (se_sort.hip) ----------------------------------------------------
+// Stable LSD radix sort of (k0, v0) by the low `bits` bits of the (non-negative) keys,
+// using (k1, v1) as the second buffer pair; *keys_out / *vals_out point at the sorted keys
+// and values (one of the two pairs).  inverse: *vals_out holds the inverse permutation
+// instead, (*vals_out)[v] = sorted position of the pair with value v (values must be a
+// permutation of [0, n)).  tmp holds sort_tmp_bytes(n) bytes or more.  Launches only
+// kernels (no memset), so it can be graph-captured.
+size_t sort_tmp_bytes(int n_cap);
+void stable_sort_pairs(int* k0, int* v0, int* k1, int* v1, int n, int bits, bool inverse, void* tmp,
+                       size_t tmp_bytes, hipStream_t s, const int** keys_out, int** vals_out);
+// count of every key from sorted keys (count zero before): counters at count_slot(k, nslots)
+// if nslots > 0, else at k
+void launch_run_counts(const int* skey, int n, int nslots, int* count, hipStream_t s);
+
+// bucketing: key[n] and a provisional rank inside the key from the counter atomics, counts
+// per key; after the scan of the counts, launch_stable_positions gives each particle's
+// position pos[n] in (key, index) order (memb: N ints of scratch), where the scatter
+// writes record n
 template <typename T>
 void launch_bin_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key,
                      int* rank, int* count, hipStream_t s);
+void launch_stable_positions(const int* key, const int* rank, const int* row_start, int N, int nb, int* memb,
+                             int* pos, hipStream_t s);
 void launch_tile_chunks(const int* row_start, int ntiles, int K, int chunk, int* tile_count,
                         int* nchunk, hipStream_t s);
 void launch_items(const int* tile_count, const int* row_start, const int* chunk_start, int ntiles,
@@ -249,15 +310,15 @@ void launch_items(const int* tile_count, const int* row_start, const int* chunk_
                   hipStream_t s);
 template <typename T>
 void launch_scatter(const double* x, const double* q, int N, const Geom& g, const Tiling& tl,
-                    const int* key, const int* rank, const int* row_start, Rec<T>* rec,
-                    hipStream_t s);
-// grouped forms: `blocks` = sum over systems of ceil(N/256); discard bucket S * nbins
+                    const int* pos, Rec<T>* rec, hipStream_t s);
+// grouped forms: `blocks` = sum over systems of ceil(N/256); discard bucket S * nbins;
+// idx / pos are indexed by the flat particle index off + n
 template <typename T>
 void launch_bin_keys_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, int* key,
                            int* rank, int* count, hipStream_t s);
 template <typename T>
-void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, const int* key,
-                          const int* rank, const int* row_start, Rec<T>* rec, hipStream_t s);
+void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, const int* pos,
+                          Rec<T>* rec, hipStream_t s);
 // pencil kernels (se_pencil.hip): FP32 only; return false if P has no compiled pencil kernel
 bool pencil_supported(int P);
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int* item_end,

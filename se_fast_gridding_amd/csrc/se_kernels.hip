@@ -132,6 +132,8 @@ __device__ __forceinline__ int row_key(int w0, int w1, int w2, const Tiling& tl)
     return tile * tl.K + (((lx >> 2) * tl.TS + lz) * tl.TS + ly);
 }
 
+// Sort key of every particle and a provisional rank inside its key (the order of the
+// counter atomics); k_stable_rank later replaces the rank by the particle's index order.
 template <typename T>
 __global__ void k_bin_keys(const double* __restrict__ x, int N, Geom g, Tiling tl,
                            int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count)
@@ -151,11 +153,90 @@ __global__ void k_bin_keys(const double* __restrict__ x, int N, Geom g, Tiling t
 }
 
 template <typename T>
-void launch_bin_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key, int* rank,
-                     int* count, hipStream_t s)
+void launch_bin_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key, int* rank, int* count,
+                     hipStream_t s)
 {
     if (N <= 0) return;
     hipLaunchKernelGGL(k_bin_keys<T>, dim3(cdiv(N, 256)), dim3(256), 0, s, x, N, g, tl, key, rank, count);
+}
+
+// Members of every key in the order of the counter atomics: memb[row_start[k] + rank[n]] = n.
+__global__ void k_members(const int* __restrict__ key, const int* __restrict__ rank, const int* __restrict__ row_start,
+                          int N, int* __restrict__ memb)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) memb[row_start[key[n]] + rank[n]] = n;
+}
+
+// Stable position of every particle: the key's first row plus the number of members of the
+// key with a smaller index, so the order inside a key is the index order whatever the
+// order of the counter atomics.  One thread per member position: the threads of one key
+// read the same members at the same time (a broadcast), and a key of s members costs
+// s steps per thread.  Keys of more than STABLE_RANK_MAX members (over 16k particles per
+// grid cell) keep the atomics' order, which bounds the cost at STABLE_RANK_MAX steps per
+// particle.
+// A work-group whose 256 positions all lie in one key (the dense keys of clustered input)
+// stages the key's members through LDS, 256 at a time, and every thread compares against
+// them by broadcast LDS reads.
+constexpr int STABLE_RANK_MAX = 1 << 16;
+constexpr int STABLE_RANK_THREADS = 256;
+__global__ __launch_bounds__(STABLE_RANK_THREADS) void k_stable_rank(const int* __restrict__ memb,
+                                                                     const int* __restrict__ key,
+                                                                     const int* __restrict__ row_start, int N, int nb,
+                                                                     int* __restrict__ pos)
+{
+    __shared__ __align__(16) int stage[STABLE_RANK_THREADS];
+    __shared__ int bounds[2];
+    const int tid = threadIdx.x, i0 = blockIdx.x * STABLE_RANK_THREADS, i = i0 + tid;
+    int n = -1, a = 0, b = 0;
+    if (i < N) {
+        n = memb[i];
+        const int k = key[n];
+        a = row_start[k];
+        b = k < nb ? row_start[k + 1] : N;
+    }
+    // the group shares one key iff its first and last positions share one
+    const int il = min(i0 + STABLE_RANK_THREADS, N) - 1;
+    if (i == i0) bounds[0] = a;
+    if (i == il) bounds[1] = b;
+    __syncthreads();
+    const int A = bounds[0], B = bounds[1];   // block-uniform
+    const bool one_key = A == a && B == b && i0 >= a && il < b;
+    if (__syncthreads_and(i >= N || one_key) && B - A <= STABLE_RANK_MAX) {
+        // every thread (also those past N) runs the same staged loop over [A, B)
+        int r = 0;
+        for (int j0 = A; j0 < B; j0 += STABLE_RANK_THREADS) {
+            stage[tid] = j0 + tid < B ? memb[j0 + tid] : 0x7fffffff;
+            __syncthreads();
+            const int4* st4 = reinterpret_cast<const int4*>(stage);
+#pragma unroll 8
+            for (int t = 0; t < STABLE_RANK_THREADS / 4; ++t) {
+                const int4 v = st4[t];
+                r += (v.x < n) + (v.y < n) + (v.z < n) + (v.w < n);
+            }
+            __syncthreads();
+        }
+        if (i < N) pos[n] = A + r;
+        return;
+    }
+    if (i >= N) return;
+    int r = i - a;
+    if (b - a <= STABLE_RANK_MAX) {
+        r = 0;
+        int j = a;
+        for (; j + 4 <= b; j += 4) r += (memb[j] < n) + (memb[j + 1] < n) + (memb[j + 2] < n) + (memb[j + 3] < n);
+        for (; j < b; ++j) r += memb[j] < n;
+    }
+    pos[n] = a + r;
+}
+
+void launch_stable_positions(const int* key, const int* rank, const int* row_start, int N, int nb, int* memb,
+                             int* pos, hipStream_t s)
+{
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_members, dim3(cdiv(N, 256)), dim3(256), 0, s, key, rank, row_start, N, memb);
+    hipLaunchKernelGGL(k_stable_rank, dim3(cdiv(N, STABLE_RANK_THREADS)), dim3(STABLE_RANK_THREADS), 0, s, memb, key,
+                       row_start, N, nb, pos);
 }
 
 // Per tile: particle count (from the row-key starts) and number of work items.
@@ -204,10 +285,10 @@ void launch_items(const int* tile_count, const int* row_start, const int* chunk_
                        ntiles, K, chunk, item_tile, item_beg, item_end, n_items);
 }
 
+// Records in sorted order: particle n goes to its sorted position pos[n].
 template <typename T>
 __global__ void k_scatter(const double* __restrict__ x, const double* __restrict__ q, int N, Geom g,
-                          Tiling tl, const int* __restrict__ key, const int* __restrict__ rank,
-                          const int* __restrict__ row_start, Rec<T>* __restrict__ rec)
+                          Tiling tl, const int* __restrict__ pos, Rec<T>* __restrict__ rec)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -216,7 +297,6 @@ __global__ void k_scatter(const double* __restrict__ x, const double* __restrict
     int w0 = window_start(x[n], g, t0, ok);
     int w1 = window_start(x[n + (long)N], g, t1, ok);
     int w2 = window_start_z(x[n + 2l * N], g, t2, ok);
-    const int k = key[n];
     Rec<T> r;
     r.t[0] = (T)t0;
     r.t[1] = (T)t1;
@@ -224,16 +304,15 @@ __global__ void k_scatter(const double* __restrict__ x, const double* __restrict
     r.q = (T)q[n];
     r.lo = (w0 % tl.TS) | ((w1 % tl.TS) << 8) | ((w2 % tl.TS) << 16);
     r.orig = n;
-    rec[row_start[k] + rank[n]] = r;
+    rec[pos[n]] = r;
 }
 
 template <typename T>
-void launch_scatter(const double* x, const double* q, int N, const Geom& g, const Tiling& tl, const int* key,
-                    const int* rank, const int* row_start, Rec<T>* rec, hipStream_t s)
+void launch_scatter(const double* x, const double* q, int N, const Geom& g, const Tiling& tl, const int* pos,
+                    Rec<T>* rec, hipStream_t s)
 {
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_scatter<T>, dim3(cdiv(N, 256)), dim3(256), 0, s, x, q, N, g, tl, key, rank,
-                       row_start, rec);
+    hipLaunchKernelGGL(k_scatter<T>, dim3(cdiv(N, 256)), dim3(256), 0, s, x, q, N, g, tl, pos, rec);
 }
 
 // Grouped bucketing: block b belongs to the last system whose first block is <= b
@@ -277,8 +356,7 @@ void launch_bin_keys_group(const SysDesc* sys, int S, int blocks, const Geom& g,
 
 template <typename T>
 __global__ void k_scatter_group(const SysDesc* __restrict__ sys, int S, Geom g, Tiling tl,
-                                const int* __restrict__ key, const int* __restrict__ rank,
-                                const int* __restrict__ row_start, Rec<T>* __restrict__ rec)
+                                const int* __restrict__ pos, Rec<T>* __restrict__ rec)
 {
     const int si = block_system(sys, S, blockIdx.x);
     const SysDesc d = sys[si];
@@ -289,7 +367,7 @@ __global__ void k_scatter_group(const SysDesc* __restrict__ sys, int S, Geom g, 
     const int w0 = window_start(d.x[n], g, t0, ok);
     const int w1 = window_start(d.x[n + (long)d.N], g, t1, ok);
     const int w2 = window_start_z(d.x[n + 2l * d.N], g, t2, ok);
-    const int f = d.off + n, k = key[f];
+    const int f = d.off + n;
     Rec<T> r;
     r.t[0] = (T)t0;
     r.t[1] = (T)t1;
@@ -297,16 +375,15 @@ __global__ void k_scatter_group(const SysDesc* __restrict__ sys, int S, Geom g, 
     r.q = (T)d.q[n];
     r.lo = (w0 % tl.TS) | ((w1 % tl.TS) << 8) | ((w2 % tl.TS) << 16);
     r.orig = n;   // index inside the system; the gather finds the system from the tile
-    rec[row_start[k] + rank[f]] = r;
+    rec[pos[f]] = r;
 }
 
 template <typename T>
-void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, const int* key,
-                          const int* rank, const int* row_start, Rec<T>* rec, hipStream_t s)
+void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, const int* pos,
+                          Rec<T>* rec, hipStream_t s)
 {
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_scatter_group<T>, dim3(blocks), dim3(256), 0, s, sys, S, g, tl, key, rank, row_start,
-                       rec);
+    hipLaunchKernelGGL(k_scatter_group<T>, dim3(blocks), dim3(256), 0, s, sys, S, g, tl, pos, rec);
 }
 
 // ================================================================================
@@ -339,10 +416,14 @@ __global__ __launch_bounds__(512) void k_spread_tiles(
     const int item = blockIdx.x;
     if (item >= *n_items) return;   // uniform across the work-group
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ float wmax[8];
     double* sub = reinterpret_cast<double*>(smem_raw);   // FP64 accumulation (ds_add_f64)
     const int P = PT > 0 ? PT : g.P;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    const int beg = item_beg[item], end = item_end[item];
+    constexpr bool FIX = sizeof(T) == 4;   // FP32: integer-valued (deterministic) LDS sums
     for (int e = tid; e < tl.sub_elems; e += nthr) sub[e] = 0.0;
+    if (FIX) item_qmax_part(rec, beg, end, tid, nthr, wmax);
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
@@ -351,8 +432,9 @@ __global__ __launch_bounds__(512) void k_spread_tiles(
 #pragma unroll
     for (int u = 0; u < UM; ++u) off[u] = lc.j(u) * tl.Szp + lc.k(u);
     const int slab = tl.S * tl.Szp;
-    const T h = (T)g.h, c = (T)g.c, dn = (T)g.d;
-    const int beg = item_beg[item], end = item_end[item];
+    // FP32: fixed-point scale of the item's LDS sums (flush_scale), folded into q d
+    const int sc = FIX ? flush_scale((double)(end - beg) * item_qmax_read(wmax, nw) * g.d, g.d) : 0;
+    const T h = (T)g.h, c = (T)g.c, dn = (T)ldexp(g.d, sc);
     for (int p = beg + wave; p < end; p += nw) {
         const Rec<T> r = rec[p];
         T fx = 0, fy = 0, fz = 0;
@@ -377,7 +459,7 @@ __global__ __launch_bounds__(512) void k_spread_tiles(
             double* row = base + i * slab;
 #pragma unroll
             for (int u = 0; u < UM; ++u)
-                if (lc.ok(u)) lds_add(row + off[u], (double)(xi * yz[u]));
+                if (lc.ok(u)) lds_add(row + off[u], FIX ? (double)rintf((float)(xi * yz[u])) : (double)(xi * yz[u]));
         }
     }
     __syncthreads();
@@ -385,7 +467,8 @@ __global__ __launch_bounds__(512) void k_spread_tiles(
     const int S = tl.S;
     SubWalk w(S, tid, nthr);   // a = ix, b = iy, c = iz
     for (int e = tid; e < tl.part_elems; e += nthr) {
-        out[e] = (T)sub[(w.a * S + w.b) * tl.Szp + w.c];
+        const double v = sub[(w.a * S + w.b) * tl.Szp + w.c];
+        out[e] = (T)(FIX ? ldexp(v, -sc) : v);
         w.next();
     }
 }
@@ -1337,11 +1420,11 @@ void launch_reorder_gather(const double* xin, const double* qin, const int* perm
     template void launch_bin_keys<T>(const double*, int, const Geom&, const Tiling&, int*, int*, int*,          \
                                      hipStream_t);                                                               \
     template void launch_scatter<T>(const double*, const double*, int, const Geom&, const Tiling&, const int*,  \
-                                    const int*, const int*, Rec<T>*, hipStream_t);                               \
+                                    Rec<T>*, hipStream_t);                                                       \
     template void launch_bin_keys_group<T>(const SysDesc*, int, int, const Geom&, const Tiling&, int*, int*,    \
                                            int*, hipStream_t);                                                   \
     template void launch_scatter_group<T>(const SysDesc*, int, int, const Geom&, const Tiling&, const int*,     \
-                                          const int*, const int*, Rec<T>*, hipStream_t);                         \
+                                          Rec<T>*, hipStream_t);                                                 \
     template void launch_spread_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \
                                          const Geom&, const Tiling&, T*, hipStream_t);                           \
     template void launch_merge<T>(const T*, const int*, const int4*, const int*, const int4*, const int*, int,  \

@@ -222,10 +222,13 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
     const int item = blockIdx.x;
     if (item >= *n_items) return;   // uniform across the work-group
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ float wmax[spread_pencil_max_threads(P) / 64];
     double* sub = reinterpret_cast<double*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    const int beg = item_beg[item], end = item_end[item], n = end - beg;
     for (int e = tid; e < S3; e += nthr) sub[e] = 0.0;
+    item_qmax_part(rec, beg, end, tid, nthr, wmax);
     __syncthreads();
 
     // wave index made provably uniform, so the particle range and all ring bookkeeping
@@ -234,9 +237,12 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
     int pi[R], pk[R], pofs[R], bx[R], bz[R];
     lane_pairs<P, R>(lane, S, pi, pk, pofs, bx, bz);
 
-    const int beg = item_beg[item], end = item_end[item], n = end - beg;
     const int wb = beg + (int)(((long)n * wave) / nw), we = beg + (int)(((long)n * (wave + 1)) / nw);
-    const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
+    // fixed-point scale of the item's LDS sums (flush_scale): q d 2^sc is folded into the
+    // x factor, each flushed value is rounded to an integer, the write-out scales back
+    const int sc = __builtin_amdgcn_readfirstlane(flush_scale((double)n * item_qmax_read(wmax, nw) * g.d, g.d));
+    const float h = (float)g.h, c = (float)g.c;
+    const float dn = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)ldexp(g.d, sc))));
 
     float acc[P][R];
 #pragma unroll
@@ -258,8 +264,10 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
                 double* rowp = sub + rho * S2 + pbase;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    // padding slots hold 0 and add it to a valid, distinct location
-                    __hip_atomic_fetch_add(rowp + pofs[r], (double)acc[s][r], __ATOMIC_RELAXED,
+                    // padding slots hold 0 and add it to a valid, distinct location; the
+                    // value is rounded to an integer (in FP32: a float of 2^23 or more is
+                    // one already), so the FP64 sums are exact
+                    __hip_atomic_fetch_add(rowp + pofs[r], (double)rintf(acc[s][r]), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                     acc[s][r] = 0.f;
                 }
@@ -349,7 +357,7 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
     float* out = partials + (size_t)item * tl.part_elems;
     SubWalk w(S, tid, nthr);   // canonical [x][y][z]: a = ix, b = iy, c = iz
     for (int e = tid; e < tl.part_elems; e += nthr) {
-        out[e] = (float)sub[(w.b * S + w.a) * S + w.c];
+        out[e] = (float)ldexp(sub[(w.b * S + w.a) * S + w.c], -sc);
         w.next();
     }
 }

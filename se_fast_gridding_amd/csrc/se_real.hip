@@ -44,8 +44,11 @@ __device__ __forceinline__ bool real_cell(const RealGeom& g, int d, double v, do
     return true;
 }
 
+// Cell of every particle (discarded particles: cell ncells, sorted last).  The order inside
+// a cell and the per-cell counts come from a stable radix sort of (cell, index) afterwards,
+// so each particle's neighbour sum runs in the same order on every call.
 __global__ void k_real_bin(const double* __restrict__ x, int N, RealGeom g, int* __restrict__ cell,
-                           int* __restrict__ rank, int* __restrict__ count)
+                           int* __restrict__ idx)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -55,21 +58,20 @@ __global__ void k_real_bin(const double* __restrict__ x, int N, RealGeom g, int*
     for (int d = 0; d < 3; ++d) ok = real_cell(g, d, x[n + (long)d * N], w, c[d]) && ok;
     const int k = ok ? (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2] : g.ncells;
     cell[n] = k;
-    rank[n] = atomicAdd(&count[k], 1);
+    idx[n] = n;
 }
 
-__global__ void k_real_scatter(const double* __restrict__ x, const double* __restrict__ q, int N, RealGeom g,
-                               const int* __restrict__ cell, const int* __restrict__ rank,
-                               const int* __restrict__ start, double4* __restrict__ pos, int* __restrict__ orig)
+// Sorted position i holds particle sidx[i]: its folded position and charge.
+__global__ void k_real_gather(const double* __restrict__ x, const double* __restrict__ q, int N, RealGeom g,
+                              const int* __restrict__ scell, const int* __restrict__ sidx,
+                              double4* __restrict__ pos, int* __restrict__ orig)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const int k = cell[n];
-    if (k >= g.ncells) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || scell[i] >= g.ncells) return;
+    const int n = sidx[i];
     double w[3];
     int c;
     for (int d = 0; d < 3; ++d) real_cell(g, d, x[n + (long)d * N], w[d], c);
-    const int i = start[k] + rank[n];
     pos[i] = make_double4(w[0], w[1], w[2], q[n]);
     orig[i] = n;
 }
@@ -161,7 +163,48 @@ __global__ void k_real_eval(const double4* __restrict__ pos, const int* __restri
     }
 }
 
-void launch_fill_i32(int* p, long n, int v, hipStream_t s);
+// Device buffers of one real-space solve for up to n particles and nc cells.
+struct RealBufs {
+    int *cell, *idx, *scell, *sidx, *count, *start, *orig;
+    double4* pos;
+    void* tmp;          // temporary area of the count scan and the cell sort
+    size_t tmp_bytes;
+};
+
+static int cell_bits(int ncells)
+{
+    int b = 1;
+    while ((1l << b) <= ncells) ++b;
+    return b;
+}
+
+// Temporary bytes the scan over nc + 1 counts and the sort of n pairs need (host-side
+// queries only: nothing is allocated or launched).
+static size_t real_tmp_bytes(int n, int nc, hipStream_t s)
+{
+    size_t a = 0;
+    SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, a, (int*)nullptr, (int*)nullptr, nc + 1, s));
+    return std::max(a, sort_tmp_bytes(n));
+}
+
+// The solve on given buffers: bin, stable sort, counts and their scan, gather, evaluate.
+static void real_run(const RealGeom& g, const RealBufs& b, int N, const double* x, const double* q, double* phi,
+                     double* force, int accumulate, hipStream_t s)
+{
+    const int nb = (N + 255) / 256;
+    launch_fill_i32(b.count, (long)g.ncells + 1, 0, s);
+    hipLaunchKernelGGL(k_real_bin, dim3(nb), dim3(256), 0, s, x, N, g, b.cell, b.idx);
+    const int* scell = b.cell;
+    int* sidx = b.idx;
+    stable_sort_pairs(b.cell, b.idx, b.scell, b.sidx, N, cell_bits(g.ncells), false, b.tmp, b.tmp_bytes, s, &scell,
+                      &sidx);
+    launch_run_counts(scell, N, 0, b.count, s);
+    size_t tb = b.tmp_bytes;
+    SE_HIP(hipcub::DeviceScan::ExclusiveSum(b.tmp, tb, b.count, b.start, g.ncells + 1, s));
+    hipLaunchKernelGGL(k_real_gather, dim3(nb), dim3(256), 0, s, x, q, N, g, scell, sidx, b.pos, b.orig);
+    hipLaunchKernelGGL(k_real_eval, dim3(nb), dim3(256), 0, s, b.pos, b.orig, b.start, g, N, phi, force, accumulate);
+    SE_HIP(hipGetLastError());
+}
 
 }  // namespace se
 
@@ -172,10 +215,7 @@ struct se_real_plan {
     RealGeom g{};
     int n_cap = 0;
     hipStream_t stream = nullptr;
-    int *cell = nullptr, *rank = nullptr, *count = nullptr, *start = nullptr, *orig = nullptr;
-    double4* pos = nullptr;
-    void* cub_tmp = nullptr;
-    size_t cub_bytes = 0;
+    RealBufs b{};
     std::vector<void*> allocs;
     void* alloc(size_t b)
     {
@@ -249,15 +289,17 @@ int se_real_plan_create(se_real_plan** out, const se_real_desc* d, int n_max, vo
         pl->n_cap = std::max(n_max, 1);
         pl->stream = (hipStream_t)stream;
         const size_t N = pl->n_cap, nc = pl->g.ncells;
-        pl->cell = (int*)pl->alloc(N * 4);
-        pl->rank = (int*)pl->alloc(N * 4);
-        pl->orig = (int*)pl->alloc(N * 4);
-        pl->pos = (double4*)pl->alloc(N * sizeof(double4));
-        pl->count = (int*)pl->alloc((nc + 1) * 4);
-        pl->start = (int*)pl->alloc((nc + 2) * 4);
-        SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->count, pl->start, (int)(nc + 1),
-                                                pl->stream));
-        pl->cub_tmp = pl->alloc(pl->cub_bytes);
+        RealBufs& b = pl->b;
+        b.cell = (int*)pl->alloc(N * 4);
+        b.idx = (int*)pl->alloc(N * 4);
+        b.scell = (int*)pl->alloc(N * 4);
+        b.sidx = (int*)pl->alloc(N * 4);
+        b.orig = (int*)pl->alloc(N * 4);
+        b.pos = (double4*)pl->alloc(N * sizeof(double4));
+        b.count = (int*)pl->alloc((nc + 1) * 4);
+        b.start = (int*)pl->alloc((nc + 2) * 4);
+        b.tmp_bytes = real_tmp_bytes((int)N, (int)nc, pl->stream);
+        b.tmp = pl->alloc(b.tmp_bytes);
     });
     if (rc != SE_OK) {
         delete pl;
@@ -282,18 +324,7 @@ int se_real_solve(se_real_plan* pl, int N, const double* x, const double* q, dou
         SE_REQUIRE(N >= 0 && N <= pl->n_cap, SE_ERR_CAPACITY, "N exceeds the real-space plan's capacity");
         SE_REQUIRE(N == 0 || (x && q), SE_ERR_ARG, "null particle arrays");
         if (N == 0 || (!phi && !force)) return;
-        const RealGeom& g = pl->g;
-        hipStream_t s = pl->stream;
-        const int nb = (N + 255) / 256;
-        launch_fill_i32(pl->count, (long)g.ncells + 1, 0, s);
-        hipLaunchKernelGGL(k_real_bin, dim3(nb), dim3(256), 0, s, x, N, g, pl->cell, pl->rank, pl->count);
-        size_t tb = pl->cub_bytes;
-        SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->count, pl->start, g.ncells + 1, s));
-        hipLaunchKernelGGL(k_real_scatter, dim3(nb), dim3(256), 0, s, x, q, N, g, pl->cell, pl->rank, pl->start,
-                           pl->pos, pl->orig);
-        hipLaunchKernelGGL(k_real_eval, dim3(nb), dim3(256), 0, s, pl->pos, pl->orig, pl->start, g, N, phi, force,
-                           pl->desc.accumulate);
-        SE_HIP(hipGetLastError());
+        real_run(pl->g, pl->b, N, x, q, phi, force, pl->desc.accumulate, pl->stream);
     });
 }
 

@@ -1,0 +1,167 @@
+// Stable LSD radix sort of (key, value) int pairs, used by the bucketing of the real-space
+// cell lists: the order of equal keys is the input order, so the particle order inside a
+// cell does not depend on the timing of any atomic.  (The engine's bucketing keeps its
+// counting sort and fixes the order inside each key afterwards, which costs less at the
+// engine's key counts; see k_stable_rank.)
+//
+// Each pass sorts by one 8-bit digit in three launches:
+//   k_sort_hist     per tile of 4096 items, the count of each digit (LDS counters)
+//   (scan)          exclusive scan of the digit-major table hist[digit * ntile + tile]
+//   k_sort_scatter  per tile, each item's position = the scanned base of (digit, tile) +
+//                   the number of earlier items of the tile with the same digit.
+// The scatter walks its tile in 16 rounds of 256 items; inside a round, a wave finds the
+// lanes that share a lane's digit with 8 ballots (one per digit bit) and takes the lane's
+// rank among them from a popcount, the waves' per-digit counts are combined in LDS in wave
+// order, and a per-digit running count carries the rounds.  No atomics order anything and
+// nothing is cleared with a memset, so the sort can be captured in a HIP graph.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "se_internal.h"
+
+namespace se {
+
+namespace {
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_WAVES = SORT_THREADS / 64;
+constexpr int SORT_ROUNDS = 16;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;   // items per tile
+constexpr int SORT_RADIX = 256;
+
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const int* __restrict__ keys, int n, int shift, int ntile,
+                                                            int* __restrict__ hist)
+{
+    __shared__ int cnt[SORT_RADIX];
+    const int tid = threadIdx.x;
+    cnt[tid] = 0;
+    __syncthreads();
+    const long base = (long)blockIdx.x * SORT_TILE;
+#pragma unroll 4
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        const long i = base + r * SORT_THREADS + tid;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (SORT_RADIX - 1)], 1);
+    }
+    __syncthreads();
+    hist[(long)tid * ntile + blockIdx.x] = cnt[tid];
+}
+
+// kout / vout: sorted pairs.  inv != null (last pass): inv[v] = the item's sorted position
+// instead of vout.
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(const int* __restrict__ kin, const int* __restrict__ vin,
+                                                               int n, int shift, int ntile,
+                                                               const int* __restrict__ offs, int* __restrict__ kout,
+                                                               int* __restrict__ vout, int* __restrict__ inv)
+{
+    __shared__ int run[SORT_RADIX];               // next free position of each digit
+    __shared__ int wcnt[SORT_WAVES][SORT_RADIX];  // this round's count of each digit per wave
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    run[tid] = offs[(long)tid * ntile + blockIdx.x];
+#pragma unroll
+    for (int v = 0; v < SORT_WAVES; ++v) wcnt[v][tid] = 0;
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;   // lanes below this one
+    const long base = (long)blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ROUNDS; ++r) {
+        const long i = base + r * SORT_THREADS + tid;
+        const bool ok = i < n;
+        const int k = ok ? kin[i] : 0;
+        const int v = ok ? vin[i] : 0;
+        const int d = (k >> shift) & (SORT_RADIX - 1);
+        // lanes of this wave with the same digit (valid lanes only)
+        unsigned long long m = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long bb = __ballot((d >> b) & 1);
+            m &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        const int below = __popcll(m & lt);
+        if (ok && below == 0) wcnt[w][d] = __popcll(m);   // the group's lowest lane
+        __syncthreads();
+        if (ok) {
+            int p = run[d] + below;
+            for (int u = 0; u < w; ++u) p += wcnt[u][d];
+            kout[p] = k;
+            if (inv)
+                inv[v] = p;
+            else
+                vout[p] = v;
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int u = 0; u < SORT_WAVES; ++u) {
+            add += wcnt[u][tid];
+            wcnt[u][tid] = 0;
+        }
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+int sort_tiles(int n) { return (int)((n + (long)SORT_TILE - 1) / SORT_TILE); }
+
+size_t sort_tmp_bytes(int n_cap)
+{
+    const int nt = std::max(1, sort_tiles(std::max(n_cap, 1)));
+    size_t b = 0;
+    SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, (int*)nullptr, (int*)nullptr, SORT_RADIX * nt));
+    const size_t hist = (size_t)2 * SORT_RADIX * nt * sizeof(int);
+    return ((hist + 255) & ~(size_t)255) + b;
+}
+
+void stable_sort_pairs(int* k0, int* v0, int* k1, int* v1, int n, int bits, bool inverse, void* tmp,
+                       size_t tmp_bytes, hipStream_t s, const int** keys_out, int** vals_out)
+{
+    *keys_out = k0;
+    *vals_out = v0;
+    if (n <= 0) return;
+    const int nt = sort_tiles(n);
+    const size_t hist_bytes = (size_t)2 * SORT_RADIX * nt * sizeof(int);
+    const size_t hist_pad = (hist_bytes + 255) & ~(size_t)255;
+    SE_REQUIRE(tmp_bytes >= hist_pad, SE_ERR_CAPACITY, "sort temporary area too small");
+    int* hist = reinterpret_cast<int*>(tmp);
+    int* offs = hist + (size_t)SORT_RADIX * nt;
+    void* scan_tmp = reinterpret_cast<char*>(tmp) + hist_pad;
+    const int passes = (std::max(bits, 1) + 7) / 8;
+    int *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        const bool last = p == passes - 1;
+        hipLaunchKernelGGL(k_sort_hist, dim3(nt), dim3(SORT_THREADS), 0, s, ki, n, shift, nt, hist);
+        size_t sb = tmp_bytes - hist_pad;
+        SE_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, sb, hist, offs, SORT_RADIX * nt, s));
+        // the last pass of an inverse sort writes the inverse permutation into the values
+        // output (its sorted values are not needed)
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(SORT_THREADS), 0, s, ki, vi, n, shift, nt, offs, ko, vo,
+                           (last && inverse) ? vo : nullptr);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    *keys_out = ki;   // the last pass's output
+    *vals_out = vi;
+}
+
+// Per-key counts from the sorted keys: each run of equal keys [a, b) adds -a at its first
+// and +b at its last item, two integer atomics per key, so the count b - a does not depend
+// on their order; count must be zero before.
+// nslots > 0: the counters sit at count_slot(k, nslots); 0: at k.
+__global__ void k_run_counts(const int* __restrict__ skey, int n, int nslots, int* __restrict__ count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = skey[i];
+    const int c = nslots > 0 ? count_slot(k, nslots) : k;
+    if (i == 0 || skey[i - 1] != k) atomicAdd(&count[c], -i);
+    if (i == n - 1 || skey[i + 1] != k) atomicAdd(&count[c], i + 1);
+}
+
+void launch_run_counts(const int* skey, int n, int nslots, int* count, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_run_counts, dim3((n + 255) / 256), dim3(256), 0, s, skey, n, nslots, count);
+}
+
+}  // namespace se

@@ -12,7 +12,11 @@ from conftest import rms_rel
 pytestmark = pytest.mark.gpu
 
 DEV = torch.device("cuda:0")
-FP32_TOL = 1e-5     # rms-relative, FP32 spread/FFT/gather vs FP64 oracle
+# rms-relative, FP32 spread/FFT/gather vs the FP64 oracle at the same P: potential 1e-6
+# (measured 1e-7..3e-7); the force carries one more FP32 rounding (its derivative factors)
+# and measured up to 1.04e-6 at P = 8 on clustered input, so it has 2e-6
+FP32_TOL = 1e-6
+FP32_TOL_F = 2e-6
 
 
 def system(N, seed, clustered=False):
@@ -75,7 +79,7 @@ def test_fp32_solve_matches_oracle(oracle, case):
     e_phi = rms_rel(phi, oracle.se3p(x, q, 1.0, M, P, m, xi))
     e_F = rms_rel(F, oracle.se3p(x, q, 1.0, M, P, m, xi, force=True))
     print("fp32 rms-rel phi %.2e F %.2e" % (e_phi, e_F))
-    assert e_phi < FP32_TOL and e_F < FP32_TOL
+    assert e_phi < FP32_TOL and e_F < FP32_TOL_F
 
 
 def test_spread_grid_matches_oracle_wrap(oracle):
@@ -206,7 +210,7 @@ def test_pencil_kernels_match_oracle_and_generic(oracle, P):
     for kw in (dict(), dict(tile=4, chunk=37), dict(tile=6, chunk=300, block=256), dict(tile=10, block=64)):
         phi, F, info = solve(x, q, M, P, xi, se.PREC_FP32, **kw)
         assert info["pencil"] == 1, kw
-        assert rms_rel(phi, ref_phi) < FP32_TOL and rms_rel(F, ref_F) < FP32_TOL, kw
+        assert rms_rel(phi, ref_phi) < FP32_TOL and rms_rel(F, ref_F) < FP32_TOL_F, kw
         assert rms_rel(phi, gphi) < 2e-6 and rms_rel(F, gF) < 2e-6, kw
     # potential-only and force-only gathers agree with the combined one
     plan = se.Plan(M, P, 1.0, xi, N, precision=se.PREC_FP32)
@@ -217,7 +221,7 @@ def test_pencil_kernels_match_oracle_and_generic(oracle, P):
     plan.interp(None, f1)
     torch.cuda.synchronize()
     plan.close()
-    assert rms_rel(p1.cpu().numpy(), ref_phi) < FP32_TOL and rms_rel(f1.cpu().numpy(), ref_F) < FP32_TOL
+    assert rms_rel(p1.cpu().numpy(), ref_phi) < FP32_TOL and rms_rel(f1.cpu().numpy(), ref_F) < FP32_TOL_F
 
 
 @pytest.mark.parametrize("clustered", [False, True])
@@ -251,7 +255,7 @@ def test_mfma_gather_matches_vector_gather(oracle, clustered):
         for u, v in zip(a, b):
             assert rms_rel(u, v) < 2e-6, kw
         assert rms_rel(a[2], a[0]) < 1e-7 and rms_rel(a[3], a[1]) < 1e-7, kw
-        assert rms_rel(a[0], ref_phi) < FP32_TOL and rms_rel(a[1], ref_F) < FP32_TOL, kw
+        assert rms_rel(a[0], ref_phi) < FP32_TOL and rms_rel(a[1], ref_F) < FP32_TOL_F, kw
 
 
 def clustered_system(N, seed=3, n_clusters=64, frac=0.8, sigma=0.01):
@@ -285,7 +289,7 @@ def test_clustered_full_size_fp32_matches_fp64():
     e_phi, e_F = rms_rel(phi32, phi64), rms_rel(F32, F64)
     print("clustered fp32 vs fp64: phi %.2e F %.2e, items %d / tiles %d" % (e_phi, e_F, info32["items"],
                                                                           info32["ntiles"]))
-    assert e_phi < FP32_TOL and e_F < FP32_TOL
+    assert e_phi < FP32_TOL and e_F < FP32_TOL_F
 
 
 def test_capacity_is_enforced():

@@ -13,7 +13,8 @@ from conftest import rms_rel
 pytestmark = pytest.mark.gpu
 
 DEV = torch.device("cuda:0")
-FP32_TOL = 1e-5
+FP32_TOL = 1e-6     # rms-relative, FP32 vs FP64 at the same P: potential
+FP32_TOL_F = 2e-6   # force (one more FP32 rounding in its derivative factors)
 
 
 def slab_system(N, seed, L=1.0, slab=1.0):
@@ -94,7 +95,7 @@ def test_se2p_fp32_matches_fp64():
     assert info32["pencil"] == 1
     e_phi, e_F = rms_rel(p32, p64), rms_rel(F32, F64)
     print("se2p fp32 vs fp64: phi %.2e F %.2e" % (e_phi, e_F))
-    assert e_phi < FP32_TOL and e_F < FP32_TOL
+    assert e_phi < FP32_TOL and e_F < FP32_TOL_F
 
 
 def test_se2p_particles_outside_the_slab_are_left_out(oracle):
