@@ -244,6 +244,24 @@ int se_real_plan_destroy(se_real_plan* plan);
 int se_real_solve(se_real_plan* plan, int N, const double* d_x, const double* d_q, double* d_phi, double* d_force);
 
 /*
+ * se_real_solve without a plan and without allocating.  No counterpart in the reference
+ * (which allocates nothing on a device); it is for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops and harnesses that forbid
+ * hipMalloc.  It computes what se_real_solve computes on a plan created with
+ * n_max = N (the same cells, so the same arrays bit for bit), asynchronously on
+ * hip_stream, and calls no hipMalloc, hipFree or allocating library routine: its cell
+ * lists, sort buffers and scan temporary live in one static device array of
+ * SE_REAL_SCRATCH_BYTES per GPU, held until the process ends.  N is limited by that
+ * array (52 bytes per particle plus 8 per cell, at most about 4 cells per particle: at
+ * least 750 000 particles fit): a larger N returns SE_ERR_CAPACITY before anything is
+ * launched.  Calls that overlap on one GPU
+ * share the array and race: serialise them (one stream, or events).
+ */
+#define SE_REAL_SCRATCH_BYTES (64u << 20)
+int se_real_solve_noalloc(const se_real_desc* desc, int N, const double* d_x, const double* d_q, double* d_phi,
+                          double* d_force, void* hip_stream);
+
+/*
  * Batched and served solves (no reference counterpart: the reference solves one system
  * per call).  An executor owns `n_slots` HIP streams and a per-slot cache of plans
  * keyed by geometry; jobs are independent systems in device memory.  se_batch_submit

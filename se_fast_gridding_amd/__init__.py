@@ -182,6 +182,7 @@ _SIG = {
     "se_real_plan_create": (C.c_int, [C.POINTER(_VP), C.POINTER(se_real_desc), C.c_int, _VP]),
     "se_real_plan_destroy": (C.c_int, [_VP]),
     "se_real_solve": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP]),
+    "se_real_solve_noalloc": (C.c_int, [C.POINTER(se_real_desc), C.c_int, _VP, _VP, _VP, _VP, _VP]),
     "se_tune": (C.c_int, [C.POINTER(se_plan_desc), C.c_int, C.c_int, C.c_int, _VP, _I, C.POINTER(C.c_float)]),
     "se_tune_save": (C.c_int, [C.c_char_p]),
     "se_tune_load": (C.c_int, [C.c_char_p]),
@@ -482,6 +483,20 @@ class RealSpace:
     def solve(self, x, q, phi=None, force=None):
         N = _check_particles("RealSpace.solve", x, q, phi, force)
         check(capi().se_real_solve(self.h, N, _ptr(x), _ptr(q), _ptr(phi), _ptr(force)), "se_real_solve")
+
+
+def real_solve_noalloc(periodicity, box, xi, rc, x, q, phi=None, force=None, self_term=False, accumulate=False,
+                       stream=None):
+    """RealSpace(...).solve without a plan and without device allocation
+    (se_real_solve_noalloc): the cell lists live in the library's static scratch array
+    (SE_REAL_SCRATCH_BYTES per GPU); calls that overlap on one GPU race."""
+    import torch
+    N = _check_particles("real_solve_noalloc", x, q, phi, force)
+    box = [float(b) for b in (box if hasattr(box, "__len__") else (box, box, box))]
+    d = se_real_desc(periodicity, (C.c_double * 3)(*box), float(xi), float(rc), int(self_term), int(accumulate))
+    st = stream if stream is not None else torch.cuda.current_stream()
+    return capi().se_real_solve_noalloc(C.byref(d), N, _ptr(x), _ptr(q), _ptr(phi), _ptr(force),
+                                        C.c_void_p(st.cuda_stream))
 
 
 _hip = None

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "../../include/se_gpu.h"
@@ -325,6 +326,89 @@ int se_real_solve(se_real_plan* pl, int N, const double* x, const double* q, dou
         SE_REQUIRE(N == 0 || (x && q), SE_ERR_ARG, "null particle arrays");
         if (N == 0 || (!phi && !force)) return;
         real_run(pl->g, pl->b, N, x, q, phi, force, pl->desc.accumulate, pl->stream);
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Plan-free, allocation-free real-space solve (se_real_solve_noalloc).  The buffers that a
+// real-space plan allocates -- cells, sort pairs, counts, cell starts, sorted positions and
+// the scan / sort temporary area -- are carved from one file-scope device array of
+// SE_REAL_SCRATCH_BYTES per GPU.  Host code never uses the array's name as an address:
+// its device address is looked up once per device with hipGetSymbolAddress, with that
+// device current, and kept per device.  Every call on one GPU shares the array, so calls
+// that overlap on one GPU race; the count table is cleared by a kernel at the start of
+// each call, every other buffer is written before it is read.
+// ---------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[SE_REAL_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up on the first call there).
+unsigned char* scratch_on_current_device()
+{
+    int dev = 0;
+    SE_HIP(hipGetDevice(&dev));
+    SE_REQUIRE(dev >= 0 && dev < SCRATCH_MAX_DEVICES, SE_ERR_ARG, "se_real_solve_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        SE_REQUIRE(p != nullptr, SE_ERR_HIP, "se_real_solve_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    return static_cast<unsigned char*>(g_scratch_ptr[dev]);
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+int se_real_solve_noalloc(const se_real_desc* d, int N, const double* x, const double* q, double* phi, double* force,
+                          void* stream)
+{
+    if (!d) return SE_ERR_ARG;
+    return guarded_real([&] {
+        SE_REQUIRE(N >= 0, SE_ERR_ARG, "se_real_solve_noalloc: N must be >= 0");
+        SE_REQUIRE(N == 0 || (x && q), SE_ERR_ARG, "null particle arrays");
+        const RealGeom g = real_geometry(d, N);
+        if (N == 0 || (!phi && !force)) return;
+        hipStream_t s = (hipStream_t)stream;
+        // layout of the scratch area (the same buffers a plan of capacity N holds)
+        const size_t n = (size_t)N, nc = (size_t)g.ncells;
+        const size_t tmp_bytes = real_tmp_bytes(N, g.ncells, s);   // host-side size queries only
+        const size_t sz[9] = {n * 4, n * 4, n * 4, n * 4, n * 4, n * sizeof(double4), (nc + 1) * 4, (nc + 2) * 4,
+                              tmp_bytes};
+        size_t off[9], total = 0;
+        for (int i = 0; i < 9; ++i) {
+            off[i] = total;
+            total += align256(sz[i]);
+        }
+        if (total > SE_REAL_SCRATCH_BYTES)
+            throw se::Error{SE_ERR_CAPACITY, "se_real_solve_noalloc: N = " + std::to_string(N) + " with " +
+                                                 std::to_string(nc) + " cells needs " + std::to_string(total) +
+                                                 " bytes of scratch, more than its SE_REAL_SCRATCH_BYTES (" +
+                                                 std::to_string((size_t)SE_REAL_SCRATCH_BYTES) + ")"};
+        unsigned char* base = scratch_on_current_device();
+        RealBufs b{};
+        b.cell = reinterpret_cast<int*>(base + off[0]);
+        b.idx = reinterpret_cast<int*>(base + off[1]);
+        b.scell = reinterpret_cast<int*>(base + off[2]);
+        b.sidx = reinterpret_cast<int*>(base + off[3]);
+        b.orig = reinterpret_cast<int*>(base + off[4]);
+        b.pos = reinterpret_cast<double4*>(base + off[5]);
+        b.count = reinterpret_cast<int*>(base + off[6]);
+        b.start = reinterpret_cast<int*>(base + off[7]);
+        b.tmp = base + off[8];
+        b.tmp_bytes = tmp_bytes;
+        real_run(g, b, N, x, q, phi, force, d->accumulate, s);
     });
 }
 
