@@ -768,6 +768,14 @@ double SE_init_params(int* M, int* P, double* m, double eps, double L, double xi
     const double c = .95;
     const int iP = *P;
     const double m_l = c * std::sqrt(PI * iP);
+    // Lindbo & Tornberg error estimates (SE_general.c:530-545); the reference's force build
+    // prints both bounds, and so does this one when the force is requested
+    const double bound_phi = std::sqrt(Q * xi * L) / L * std::exp(-PI * iP * c * c / 2);
+    const double bound_force = Q * std::sqrt(xi * L) / L / L * std::exp(-PI * iP * c * c / 2.) * 2. * PI * c * c * iP;
+    if (g_quantity & SE_FORCE) {
+        std::printf("bound for potential : %g\n", bound_phi);
+        std::printf("bound for force: %g\n", bound_force);
+    }
     const double k_inf = 2. * xi * L;
     int M_l = (int)std::ceil(2. * k_inf + 1) + 2;
     if (M_l < iP) M_l = iP;
