@@ -220,6 +220,23 @@ __global__ __launch_bounds__(STABLE_RANK_THREADS) void k_stable_rank(const int* 
         return;
     }
     if (i >= N) return;
+    // a full wave whose lanes all lie in one key reads the key's members 64 at a time, one
+    // per lane, and compares against each of them through a lane broadcast (a wave with
+    // lanes past N has lanes that returned above and hold no member: it takes the
+    // per-lane loop below)
+    const int wa = __builtin_amdgcn_readfirstlane(a), wbnd = __builtin_amdgcn_readfirstlane(b);
+    const bool wave_key = __ballot(1) == ~0ull && __ballot(a == wa && b == wbnd) == ~0ull;
+    if (wave_key && wbnd - wa <= STABLE_RANK_MAX) {
+        const int lane = threadIdx.x & 63;
+        int r = 0;
+        for (int j0 = wa; j0 < wbnd; j0 += 64) {
+            const int v = j0 + lane < wbnd ? memb[j0 + lane] : 0x7fffffff;
+#pragma unroll 16
+            for (int t = 0; t < 64; ++t) r += __builtin_amdgcn_readlane(v, t) < n;
+        }
+        pos[n] = wa + r;
+        return;
+    }
     int r = i - a;
     if (b - a <= STABLE_RANK_MAX) {
         r = 0;

@@ -131,7 +131,12 @@ __device__ __forceinline__ float wave_sum_dpp(float v)
 // only in the last register) are mapped onto pairs of the first register, so every
 // lane addresses a distinct, in-bounds location without a branch, and fetch their x
 // factor from lane 63, whose factor is always 0 (63 >= P + 3).
-template <int P, int R>
+// Pair order: XFAST puts x fastest (pair = i + (P+3) k), so the 64 lanes of a row access
+// step through x columns S apart; for the P = 16 gather with S = 27 (tile 12) the FP32
+// addresses of a register then fall in distinct LDS banks (z fastest puts four 16-wide
+// runs 27 apart, two to a bank).  The spread and the other supports keep z fastest (the
+// spread's FP64 flush has another bank pattern, and x fastest costs it registers).
+template <int P, int R, bool XFAST = false>
 __device__ __forceinline__ void lane_pairs(int lane, int S, int (&pi)[R], int (&pk)[R], int (&pofs)[R],
                                            int (&bx)[R], int (&bz)[R])
 {
@@ -142,8 +147,8 @@ __device__ __forceinline__ void lane_pairs(int lane, int S, int (&pi)[R], int (&
         const int pr0 = lane + 64 * r;
         const bool ok = pr0 < WX * WZ;
         const int pr = ok ? pr0 : pr0 % (WX * WZ);
-        pi[r] = pr / WZ;
-        pk[r] = pr - (pr / WZ) * WZ;
+        pi[r] = XFAST ? pr % WX : pr / WZ;
+        pk[r] = XFAST ? pr / WX : pr % WZ;
         pofs[r] = pi[r] * S + pk[r];
         bx[r] = 4 * (ok ? pi[r] : 63);
         bz[r] = 4 * pk[r];
@@ -399,7 +404,7 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
 
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
     int pi[R], pk[R], pofs[R], bx[R], bz[R];
-    lane_pairs<P, R>(lane, S, pi, pk, pofs, bx, bz);
+    lane_pairs<P, R, P == 16>(lane, S, pi, pk, pofs, bx, bz);
     const int wb = beg + (int)(((long)n * wave) / nw), we = beg + (int)(((long)n * (wave + 1)) / nw);
     const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
     const double h3 = g.h * g.h * g.h;
