@@ -266,9 +266,12 @@ int se_real_solve_noalloc(const se_real_desc* desc, int N, const double* d_x, co
  * per call).  An executor owns `n_slots` HIP streams and a per-slot cache of plans
  * keyed by geometry; jobs are independent systems in device memory.  se_batch_submit
  * queues a job on the slots in turn; se_batch_try_submit takes an idle slot (one that
- * already holds a plan for the job first) or returns SE_BUSY (the admission step of a
- * serving loop).  se_batch_submit replays the job through se_solve_graph (captured on the
- * first submission of the same arguments); se_batch_try_submit solves eagerly.
+ * already holds the job's captured graph first, then one that holds a plan for the job)
+ * or returns SE_BUSY (the admission step of a serving loop).  se_batch_submit replays the
+ * job through se_solve_graph (captured on the first submission of the same arguments on
+ * that slot); se_batch_try_submit replays a graph the chosen slot already holds and
+ * otherwise solves eagerly, so it never captures on a request's path (a serving loop can
+ * capture its jobs ahead of traffic by submitting each once per slot).
  * Plans of all slots together
  * stay within `hbm_budget` bytes (0 = 64 GiB); least recently used plans of idle slots
  * are freed to make room, and a job whose plan cannot fit fails with SE_ERR_CAPACITY.

@@ -11,8 +11,11 @@
 // within the executor's byte budget (least recently used plans of idle slots are freed
 // to make room), and se_batch_try_submit refuses a job when no slot is idle.
 // se_batch_submit runs a job through its plan's captured graph (se_solve_graph), so a
-// job that is submitted again costs one launch; se_batch_try_submit runs it eagerly,
-// so that no capture lands on a request's latency.  se_batch_run takes a whole set of
+// job that is submitted again costs one launch; se_batch_try_submit replays the job's
+// graph when an idle slot already holds one (one launch instead of one per kernel) and
+// otherwise runs the job eagerly, so that no capture lands on a request's latency (a
+// serving loop captures its jobs ahead of traffic, e.g. by submitting each job once per
+// slot).  se_batch_run takes a whole set of
 // jobs, groups them by geometry and output mode, and solves each group with one group
 // plan (se_solve_group: one launch per stage for all systems of the group); the groups
 // run on different slots and overlap.  Group plans are cached by geometry across calls
@@ -200,6 +203,15 @@ struct se_batch {
         return false;
     }
 
+    // the slot's plan for the job holds the job's captured graph
+    bool has_graph(int si, const se_job* job)
+    {
+        for (auto& e : slots[si].plans)
+            if (same_shape(e.key, job->shape) && e.cap >= job->N)
+                return se::plan_has_graph(e.plan, job->N, job->d_x, job->d_q, job->d_phi, job->d_force);
+        return false;
+    }
+
     se_plan* plan_for(int si, const se_job_shape& k, int N)
     {
         Slot& s = slots[si];
@@ -338,22 +350,25 @@ int se_batch_try_submit(se_batch* b, const se_job* job, int* ticket)
     if (!b || !job || !ticket) return SE_ERR_ARG;
     *ticket = -1;
     return guarded_b([&] {
-        // an idle slot that already holds a plan for the job, else any idle slot
+        // an idle slot that holds the job's captured graph (replayed: one launch), else an
+        // idle slot that holds a plan for the job, else any idle slot (both eager)
         const int n = (int)b->slots.size();
-        int fallback = -1;
+        int with_plan = -1, fallback = -1;
         for (int r = 0; r < n; ++r) {
             const int i = (b->next + r) % n;
             if (!b->idle(i)) continue;
-            if (b->has_plan(i, job->shape, job->N)) {
+            if (b->has_graph(i, job)) {
                 b->next = (i + 1) % n;
-                *ticket = b->launch(i, job, false);
+                *ticket = b->launch(i, job, true);
                 return;
             }
+            if (with_plan < 0 && b->has_plan(i, job->shape, job->N)) with_plan = i;
             if (fallback < 0) fallback = i;
         }
-        if (fallback < 0) throw se::Error{SE_BUSY, "se_batch: no idle slot"};
-        b->next = (fallback + 1) % n;
-        *ticket = b->launch(fallback, job, false);
+        const int si = with_plan >= 0 ? with_plan : fallback;
+        if (si < 0) throw se::Error{SE_BUSY, "se_batch: no idle slot"};
+        b->next = (si + 1) % n;
+        *ticket = b->launch(si, job, false);
     });
 }
 

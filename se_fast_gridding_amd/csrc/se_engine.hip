@@ -701,6 +701,17 @@ size_t se_plan_bytes_for_group(const se_plan_desc* desc, int max_systems)
 
 }  // extern "C"
 
+namespace se {
+// True when the plan holds a captured graph of se_solve with exactly these arguments.
+bool plan_has_graph(const se_plan* pl, int N, const void* x, const void* q, const void* phi, const void* force)
+{
+    if (!pl || pl->timing) return false;
+    for (const auto& e : pl->graphs)
+        if (e.N == N && e.x == x && e.q == q && e.phi == phi && e.force == force) return true;
+    return false;
+}
+}  // namespace se
+
 static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n_sys)
 {
     if (!out || !d) return SE_ERR_ARG;

@@ -272,6 +272,10 @@ struct LParams {
     int per;
 };
 
+// True when the plan holds a captured graph of se_solve with exactly these arguments
+// (se_engine.hip; the batch executor routes a served job to a slot that has one).
+bool plan_has_graph(const se_plan* pl, int N, const void* x, const void* q, const void* phi, const void* force);
+
 // ---- tuned tilings (se_tune.hip) ---------------------------------------------------
 // The tiling se_tune recorded for d's geometry and n particles per system, if any.
 bool tuned_tiling(const se_plan_desc* d, long n_per_system, int& tile, int& chunk, int& block);
