@@ -400,7 +400,7 @@ def fp64_p21_reference(systems):
     return out
 
 
-def batched(steps, slots, tune=True, quick=False, n_sys=512):
+def batched(steps, slots, tune=True, quick=False, n_sys=512, grouped_only=False):
     """512 independent systems: FP32 grouped solves (se_batch_run: one group plan per
     geometry, one launch per stage per group, groups overlapping on the slots) vs the same
     systems solved one after another on the FP64 path at their own P (plans cached per
@@ -445,6 +445,8 @@ def batched(steps, slots, tune=True, quick=False, n_sys=512):
                                                      "N": int(np.mean(ns)), "systems": len(ns)}
         tune_s = time.perf_counter() - t0
     t32, phi32, st32 = timed(se.PREC_FP32, slots, True)
+    if grouped_only:   # for kernel traces of the grouped passes alone
+        return {"solves_per_s": n_sys / t32, "ms": t32 * 1e3, "mode": "grouped (se_batch_run), FP32, alone"}
     t32s, phi32s, _ = timed(se.PREC_FP32, slots, False)
     t64, phi64, _ = timed(se.PREC_FP64, 1, False)
     if tune:
@@ -599,6 +601,8 @@ def main():
     ap.add_argument("--no-extras", dest="extras", action="store_false",
                     help="skip the uniform, batched and serving extra keys of the default line")
     ap.add_argument("--steps-batched", type=int, default=3, help="timed passes of the batched extra")
+    ap.add_argument("--grouped-only", action="store_true",
+                    help="batched: time the grouped FP32 passes only (no per-job or FP64 baselines)")
     ap.add_argument("--extra-rates", default="1000,5000,20000", help="arrival rates of the serving extra")
     ap.add_argument("--extra-requests", type=int, default=3000, help="requests per rate of the serving extra")
     ap.add_argument("--tile", type=int, default=0, help="tile side (0 = plan default)")
@@ -624,7 +628,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if args.workload == "batched":
-        out = batched(args.steps, args.slots, tune=args.tune)
+        out = batched(args.steps, args.slots, tune=args.tune, grouped_only=args.grouped_only)
         out.update({"metric": "batched solves/s", "value": out["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
                     "steps": args.steps, "higher_is_better": True, "dtype": "f32", "data": "synthetic"})
         print(json.dumps(out))

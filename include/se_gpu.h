@@ -73,6 +73,10 @@ typedef struct {
 /* Keep the P = 16 FP32 gather on the vector register-ring kernel instead of the matrix
  * cores (MFMA); for A/B comparisons and tests. */
 #define SE_PLAN_NO_MFMA 2
+/* Spread sparse tiles with the pencil register ring too, instead of the plane kernel that
+ * keeps each (x, z) column of the tile's sub-grid in registers (for A/B comparisons and
+ * tests). */
+#define SE_PLAN_NO_PLANE_SPREAD 4
 
 int se_plan_create(se_plan** plan, const se_plan_desc* desc, void* hip_stream);
 int se_plan_destroy(se_plan* plan);

@@ -56,7 +56,7 @@ struct DevMem {
 // sub-grids hold the grid precision.  Default tile side: the largest (<= 16; a multiple
 // of 4 and <= 12 for the pencil kernels) whose FP64 sub-grid fits in LDS.
 static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sys, int tile, int chunk, int block,
-                            int lds_max, bool allow_pencil, bool allow_mfma)
+                            int lds_max, bool allow_pencil, bool allow_mfma, bool allow_plane)
 {
     Tiling tl{};
     tl.pencil = (allow_pencil && esize == 4 && pencil_supported(P)) ? 1 : 0;
@@ -107,6 +107,9 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     // latency than at 512.  An explicit block applies to every kernel.
     tl.sblock = block > 0 ? block : spread_pencil_max_threads(P);
     tl.block_auto = block > 0 ? 0 : 1;
+    // plane spread of sparse tiles: one lane per (x, z) column of the sub-grid, 32 z columns
+    // per wave (S <= 32), its y values in registers (tile side <= 12)
+    tl.plane = (tl.pencil && tl.TS <= 12 && tl.S <= 32 && allow_plane) ? 1 : 0;
     // every launch shape must lie within its kernel's __launch_bounds__: refused here, at
     // plan creation, rather than by the launch guards at solve time
     SE_REQUIRE(tl.sblock <= spread_pencil_max_threads(P) && tl.mblock <= 512 && tl.block <= 512, SE_ERR_ARG,
@@ -451,7 +454,8 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
     int tile = d->tile, chunk = d->chunk, block = d->block;
     if (tile == 0 && chunk == 0 && block == 0) tuned_tiling(d, std::max(d->n_max, 1) / n_sys, tile, chunk, block);
     Tiling tl = choose_tiling(es, d->P, g.M, g.Mz, d->n_max, n_sys, tile, chunk, block, lds_max,
-                              !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA));
+                              !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA),
+                              !(d->flags & SE_PLAN_NO_PLANE_SPREAD));
     const size_t M = g.M, Mz = g.Mz, N = d->n_max, ns = n_sys;
     size_t b = 0;
     b += 4 * N * 4;                                       // key, rank, members, positions
@@ -515,8 +519,8 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int
     bool done = false;
     if constexpr (sizeof(T) == 4) {
         if (tl.pencil)
-            done = launch_spread_pencil((const Rec<float>*)pl->rec, pl->item_beg, pl->item_end, pl->n_items, pl->g,
-                                        tlc, (float*)pl->partials, s);
+            done = launch_spread_pencil((const Rec<float>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end,
+                                        pl->n_items, pl->row_start, pl->g, tlc, (float*)pl->partials, s);
     }
     if (!done)
         launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
@@ -732,7 +736,8 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         int tile = d->tile, chunk = d->chunk, block = d->block;
         if (tile == 0 && chunk == 0 && block == 0) tuned_tiling(d, pl->n_cap / n_sys, tile, chunk, block);
         pl->tl = choose_tiling(pl->esize, d->P, pl->g.M, pl->g.Mz, pl->n_cap, n_sys, tile, chunk, block, lds_max,
-                               !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA));
+                               !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA),
+                               !(d->flags & SE_PLAN_NO_PLANE_SPREAD));
         const Tiling& tl = pl->tl;
         const size_t M = pl->g.M, Mz = pl->g.Mz, N = pl->n_cap, ns = n_sys;
         SE_REQUIRE(ns * tl.nbins < (1ul << 31) - 1 && ns * tl.ntiles * (size_t)tl.K < (1ul << 31) - 1, SE_ERR_ARG,

@@ -81,6 +81,7 @@ struct Tiling {
     int mblock;       // threads per work-group of the MFMA gather
     int sblock;       // threads per work-group of the register-ring (pencil) spread
     int block_auto;   // 1: block was not given, so each kernel may take its own size
+    int plane;        // 1: sparse tiles take the plane spread (pencil spreads only)
 };
 
 // Walk over the S^3 points of a tile sub-grid, e = (a S + b) S + c, visited as e = tid,
@@ -325,9 +326,11 @@ void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, 
                           Rec<T>* rec, hipStream_t s);
 // pencil kernels (se_pencil.hip): FP32 only; return false if P has no compiled pencil kernel
 bool pencil_supported(int P);
-bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int* item_end,
-                          const int* n_items, const Geom& g, const Tiling& tl, float* partials,
-                          hipStream_t s);
+// pencil spread; where tl.plane is set, whole sparse tiles take the plane spread
+// (row_start: the sort keys' first records)
+bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
+                          const int* n_items, const int* row_start, const Geom& g, const Tiling& tl,
+                          float* partials, hipStream_t s);
 // sys != null: grouped solve, the outputs and N come from sys[tile / ntiles] and the
 // grids are stacked; phi/force then only select the output mode.  side (with the events
 // fork and join) non-null: where the items split between two kernels, the sparse items'

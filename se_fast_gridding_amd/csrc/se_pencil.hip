@@ -216,16 +216,33 @@ struct GatherRing<P, P, R, FORCE> {
 
 }  // namespace
 
+// A sparse item: a whole tile (its range is every record of the tile's keys, so it was
+// not cut into chunks) of fewer than sparse_max particles.
+__device__ __forceinline__ bool sparse_item(const int* __restrict__ item_tile, const int* __restrict__ item_beg,
+                                            const int* __restrict__ item_end, const int* __restrict__ row_start,
+                                            const Tiling& tl, int item, int sparse_max)
+{
+    if (sparse_max <= 0) return false;
+    const int beg = item_beg[item], end = item_end[item];
+    if (end - beg >= sparse_max) return false;
+    const int kb = item_tile[item] * tl.K;
+    return beg == row_start[kb] && end == row_start[kb + tl.K];
+}
+
 template <int P>
 __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(const Rec<float>* __restrict__ rec,
                                                        const int* __restrict__ item_beg,
                                                        const int* __restrict__ item_end,
                                                        const int* __restrict__ n_items, Geom g, Tiling tl,
-                                                       float* __restrict__ partials)
+                                                       float* __restrict__ partials, const int* __restrict__ item_tile,
+                                                       const int* __restrict__ row_start, int sparse_max)
 {
     constexpr int R = ((P + 3) * P + 63) / 64;
     const int item = blockIdx.x;
     if (item >= *n_items) return;   // uniform across the work-group
+    // sparse items (whole tiles of fewer than sparse_max particles) are the plane
+    // spread's (sparse_max > 0)
+    if (sparse_item(item_tile, item_beg, item_end, row_start, tl, item, sparse_max)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ float wmax[spread_pencil_max_threads(P) / 64];
     double* sub = reinterpret_cast<double*>(smem_raw);   // [y][x][z], S^3
@@ -364,6 +381,132 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
     for (int e = tid; e < tl.part_elems; e += nthr) {
         out[e] = (float)ldexp(sub[(w.b * S + w.a) * S + w.c], -sc);
         w.next();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Plane spread for sparse work items (a whole tile of fewer particles than the
+// threshold).  A 4 x 1 pencil of a sparse tile holds one or two particles, so the register
+// ring above flushes its P rows (convert, round, clear and an FP64 LDS atomic per slot) for
+// about one particle each, and the zeroing and write-out of the FP64 sub-grid cost as much
+// again.  Here lane (X, Z) of the work-group owns one (x, z) column of the S^3 sub-grid and
+// keeps all S of its y values in registers: wave w holds the columns X = 2w, 2w + 1 and
+// Z = 0 .. 31 (S <= 32).  Each particle adds w = q d Fx(X) Fz(Z) times its y factor to
+// every row of the column: the y factors are tabulated per particle at rows 0 .. 31 with
+// zeros outside the window [ly, ly + P), so every register index is static and the FMA
+// block has no branch (S FMAs per lane; the factor of row y is read from lane y mod 16 of
+// the table's first or second 16 entries with row_newbcast).  Choosing only the 4-row blocks
+// that meet the window saves FMAs but measured slower (the branches cost more).  No LDS
+// sub-grid, no atomic and no flush is needed, and the columns are stored straight from
+// registers into the item's partial.  Waves whose two x columns miss the particle's window
+// skip it.  The x and z factors are tabulated too (with zeros around the window, so a lane
+// reads its factor without a bounds test), per batch of 128 particles in LDS.  (Reading the
+// next particle's factors before the current particle's FMAs measured slower: the copies
+// cost more VALU than the waits, which the other waves cover.)  Every sum is formed in
+// FP32 in the item's record order, so the result does not depend on timing.
+// The kernel is held to 64 VGPRs (8 waves per SIMD) so that two work-groups of 14 waves
+// share a CU.
+// ---------------------------------------------------------------------------------
+constexpr int PLANE_BATCH = 128;  // particles tabulated per batch
+constexpr int PLANE_TW = 48;      // x / z table row: window index j at 16 + j, zeros around
+
+// acc[Y] += T_Y * w for Y < SM, T_Y = lane Y mod 16 of fya's (Y < 16) or fyb's (Y >= 16) row
+template <int Y, int SM>
+struct PlaneColumn {
+    static __device__ __forceinline__ void run(float (&acc)[SM], float fya, float fyb, float w)
+    {
+        if (Y < 16)
+            fmac_row_bcast<Y & 15, Y == 0>(acc[Y], fya, w);
+        else
+            fmac_row_bcast<Y & 15, Y == 16>(acc[Y], fyb, w);
+        PlaneColumn<Y + 1, SM>::run(acc, fya, fyb, w);
+    }
+};
+template <int SM>
+struct PlaneColumn<SM, SM> {
+    static __device__ __forceinline__ void run(float (&)[SM], float, float, float) {}
+};
+
+template <int P>
+__global__ __launch_bounds__(1024, 8) void k_spread_plane(const Rec<float>* __restrict__ rec,
+                                                          const int* __restrict__ item_tile,
+                                                          const int* __restrict__ item_beg,
+                                                          const int* __restrict__ item_end,
+                                                          const int* __restrict__ n_items,
+                                                          const int* __restrict__ row_start, Geom g, Tiling tl,
+                                                          float* __restrict__ partials, int sparse_max)
+{
+    constexpr int SM = 12 + P - 1;   // largest sub-grid side (tile side <= 12)
+    static_assert(SM <= 32 && P <= 16, "the y table holds 32 rows, the x and z tables 16 + 16 + 16");
+    const int item = blockIdx.x;
+    if (item >= *n_items) return;
+    if (!sparse_item(item_tile, item_beg, item_end, row_start, tl, item, sparse_max)) return;
+    __shared__ float tx[PLANE_BATCH][PLANE_TW], tz[PLANE_BATCH][PLANE_TW], ty[PLANE_BATCH][32];
+    const int beg = item_beg[item], end = item_end[item];
+    const int S = tl.S, tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int xlo = 2 * wave;                          // the wave's x columns: xlo, xlo + 1
+    const int X = xlo + (lane >> 5), Z = lane & 31;    // this lane's column
+    // the x and z tables keep zeros outside the window entries [16, 16 + P)
+    for (int e = tid; e < PLANE_BATCH * PLANE_TW; e += nthr) {
+        (&tx[0][0])[e] = 0.f;
+        (&tz[0][0])[e] = 0.f;
+    }
+
+    float acc[SM];
+#pragma unroll
+    for (int y = 0; y < SM; ++y) acc[y] = 0.f;
+    const float h = (float)g.h, kc = -(float)g.c * 1.4426950408889634f, dq = (float)g.d;
+    // byte offsets of this lane's entries for a window starting at 0 (X - lx + 16 and
+    // Z - lz + 16 stay in [16 - 11, 31 + 16] = [5, 47]); the particle's part is uniform
+    const int xo = (X + 16) * 4, zo = (Z + 16) * 4, yo = (lane & 15) * 4;
+    const char* txb = reinterpret_cast<const char*>(&tx[0][0]);
+    const char* tzb = reinterpret_cast<const char*>(&tz[0][0]);
+    const char* tyb = reinterpret_cast<const char*>(&ty[0][0]);
+    for (int b0 = beg; b0 < end; b0 += PLANE_BATCH) {
+        const int cnt = min(PLANE_BATCH, end - b0);
+        __syncthreads();   // the tables are cleared / the previous batch is consumed
+        // entry j < 16 of particle p: window index j of the x, y and z factors; the y
+        // factor goes to row ly + j, and row (ly + 16 + j) mod 32 is cleared, so the 32 rows
+        // hold the window and zeros
+        for (int e = tid; e < cnt * 16; e += nthr) {
+            const int p = e >> 4, j = e & 15;
+            const float4 r = *reinterpret_cast<const float4*>(rec + b0 + p);   // t[0], t[1], t[2], q
+            const int ly = (rec[b0 + p].lo >> 8) & 255;
+            const float o = (float)(g.first + j) * h;
+            const bool in = j < P;
+            ty[p][ly + j] = in ? gauss(kc, r.y - o) : 0.f;
+            ty[p][(ly + 16 + j) & 31] = 0.f;
+            if (in) {
+                tx[p][16 + j] = gauss(kc, r.x - o) * (r.w * dq);
+                tz[p][16 + j] = gauss(kc, r.z - o);
+            }
+        }
+        // lane l: window starts of particles b0 + l and b0 + 64 + l
+        const int lo_a = rec[b0 + min(lane, cnt - 1)].lo, lo_b = rec[b0 + min(64 + lane, cnt - 1)].lo;
+        __syncthreads();
+        for (int p = 0; p < cnt; ++p) {
+            const int la = __builtin_amdgcn_readlane(lo_a, p & 63), lb = __builtin_amdgcn_readlane(lo_b, p & 63);
+            const int lo = p < 64 ? la : lb;
+            const int lx = lo & 255;
+            if (xlo + 1 < lx || xlo >= lx + P) continue;   // both columns outside the x window
+            // uniform byte offsets of the particle's table rows, shifted by the window start
+            const int sx = __builtin_amdgcn_readfirstlane((p * PLANE_TW - lx) * 4);
+            const int sz = __builtin_amdgcn_readfirstlane((p * PLANE_TW - (lo >> 16)) * 4);
+            const int sy = __builtin_amdgcn_readfirstlane(p * 128);
+            const float w = *reinterpret_cast<const float*>(txb + (xo + sx)) *
+                            *reinterpret_cast<const float*>(tzb + (zo + sz));
+            float fya = *reinterpret_cast<const float*>(tyb + (yo + sy));
+            float fyb = *reinterpret_cast<const float*>(tyb + (yo + sy + 64));
+            asm volatile("" : "+v"(fya), "+v"(fyb));   // last definitions of the DPP sources
+            PlaneColumn<0, SM>::run(acc, fya, fyb, w);
+        }
+    }
+    if (X < S && Z < S) {   // canonical [x][y][z] partial
+        float* out = partials + (size_t)item * tl.part_elems + (size_t)X * S * S + Z;
+#pragma unroll
+        for (int y = 0; y < SM; ++y)
+            if (y < S) out[y * S] = acc[y];
     }
 }
 
@@ -844,18 +987,27 @@ static void allow_lds_p(const void* fn, size_t bytes)
     if (bytes > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-bool launch_spread_pencil(const Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items,
-                          const Geom& g, const Tiling& tl, float* partials, hipStream_t s)
+// Items of fewer particles than this (whole sparse tiles) take the plane spread, the others
+// the pencil spread; 0 turns the plane spread off.
+int spread_sparse_max(const Tiling& tl) { return tl.plane ? mfma_dense_min(tl) : 0; }
+
+bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
+                          const int* n_items, const int* row_start, const Geom& g, const Tiling& tl, float* partials,
+                          hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(double);
     SE_REQUIRE(tl.sblock >= 64 && tl.sblock % 64 == 0 && tl.sblock <= spread_pencil_max_threads(g.P), SE_ERR_ARG,
                "spread work-group size past the kernel's bound");
+    const int sparse_max = spread_sparse_max(tl);
     dim3 gr(tl.max_items), bl(tl.sblock);
 #define SE_SP(PV)                                                                                            \
     do {                                                                                                     \
         allow_lds_p((const void*)k_spread_pencil<PV>, lds);                                                  \
         hipLaunchKernelGGL(k_spread_pencil<PV>, gr, bl, lds, s, rec, item_beg, item_end, n_items, g, tl,    \
-                           partials);                                                                        \
+                           partials, item_tile, row_start, sparse_max);                                      \
+        if (sparse_max > 0)                                                                                  \
+            hipLaunchKernelGGL(k_spread_plane<PV>, gr, dim3(64 * ((tl.S + 1) / 2)), 0, s, rec, item_tile,      \
+                               item_beg, item_end, n_items, row_start, g, tl, partials, sparse_max);         \
     } while (0)
     switch (g.P) {
     case 8: SE_SP(8); return true;

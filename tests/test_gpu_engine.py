@@ -313,3 +313,32 @@ def test_selected_parameters_meet_the_tolerance_on_the_gpu(oracle, tol):
     phi, _, _ = solve(x, q, M, P, xi, se.PREC_FP64, force=False, m=m)
     ref = oracle.direct3p(x, q, (L, L, L), xi, (M - 1) // 2)
     assert np.sqrt(np.mean((phi - ref) ** 2)) <= tol
+
+
+@pytest.mark.parametrize("P", [8, 11, 16])
+def test_plane_spread_matches_pencil_spread_and_oracle(oracle, P):
+    """Sparse tiles (a few particles per 4 x 1 pencil) take the plane spread, which keeps
+    each (x, z) column of the tile's sub-grid in registers; its grid equals the pencil
+    spread's and the reference's wrapped grid, also with tiles cut into chunks (which stay
+    on the pencil kernel) and other tile sides, and it is the same on a second run."""
+    N, M, xi = 1500, 40, 6.0
+    x, q = system(N, 60 + P, clustered=True)   # dense clusters and sparse tiles in one grid
+    p = oracle.pack_params(N, M, P, plan_c(M, P, xi), 1.0 / M)
+    zs = oracle.base_gaussian(p)
+    ref = oracle.wrap(3, p, oracle.grid_split(p, q, oracle.expand_all(3, x, p), zs)).reshape(M, M, M, order="F")
+    xt, qt = torch.tensor(x, device=DEV), torch.tensor(q, device=DEV)
+    scale = np.max(np.abs(ref))
+    for kw in (dict(), dict(tile=8), dict(tile=4, chunk=25, block=128), dict(chunk=60, block=256)):
+        grids = []
+        for plane in (True, False, True):
+            plan = se.Plan(M, P, 1.0, xi, N, precision=se.PREC_FP32, plane=plane, **kw)
+            plan.spread(xt, qt)
+            torch.cuda.synchronize()
+            assert plan.info()["pencil"] == 1
+            grids.append(plan.grid().double().cpu().numpy())
+            plan.close()
+        e_plane = np.max(np.abs(grids[0] - ref)) / scale
+        e_pair = np.max(np.abs(grids[0] - grids[1])) / scale
+        print("P=%d %s: plane vs oracle %.2e, plane vs pencil %.2e" % (P, kw, e_plane, e_pair), flush=True)
+        assert e_plane < 2e-6 and e_pair < 2e-6, kw
+        assert np.array_equal(grids[0], grids[2]), kw
