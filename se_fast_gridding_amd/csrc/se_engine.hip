@@ -121,27 +121,6 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
 
 using namespace se;
 
-namespace se {
-
-// Exclusive scan of the per-key counts, read in key order from their counter slots
-// (count_slot).
-struct SlotCount {
-    const int* c;
-    int n;
-    __host__ __device__ int operator()(int k) const { return c[count_slot(k, n)]; }
-};
-
-static hipError_t scan_counts(void* tmp, size_t& bytes, const int* count, int* out, int n, hipStream_t s)
-{
-    hipcub::TransformInputIterator<int, SlotCount, hipcub::CountingInputIterator<int>> it(
-        hipcub::CountingInputIterator<int>(0), SlotCount{count, n});
-    return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, it, out, n, s);
-}
-
-
-
-}  // namespace se
-
 struct se_plan {
     se_plan_desc desc{};
     int esize = 8;
@@ -172,16 +151,17 @@ struct se_plan {
     std::vector<GroupFft> gffts;   // batched transforms, one pair per group size
     unsigned long gfft_clock = 0;
     // device buffers
-    // key / rank: sort key and provisional rank (counter atomics) per particle; memb: the
-    // members of each key; pos: each particle's position in (key, index) order
-    int *key = nullptr, *rank = nullptr, *memb = nullptr, *pos = nullptr, *count = nullptr, *row_start = nullptr, *tile_count = nullptr,
+    // key / idx and key2 / idx2: the two (sort key, particle index) buffer pairs of the
+    // stable key sort; sort_tmp: its histograms and scan area; tile_start: each tile's
+    // first sorted record
+    int *key = nullptr, *idx = nullptr, *key2 = nullptr, *idx2 = nullptr, *tile_start = nullptr, *tile_count = nullptr,
         *nchunk = nullptr, *chunk_start = nullptr, *item_tile = nullptr, *item_beg = nullptr,
         *item_end = nullptr, *n_items = nullptr, *nseg = nullptr, *nsegz = nullptr;
     int4 *seg = nullptr, *segz = nullptr;
     void *rec = nullptr, *partials = nullptr, *grid = nullptr, *spec = nullptr, *cub_tmp = nullptr,
-         *fft_work = nullptr;
+         *sort_tmp = nullptr, *fft_work = nullptr;
     double* gtab = nullptr;
-    size_t cub_bytes = 0, fft_work_bytes = 0, bytes = 0;
+    size_t cub_bytes = 0, sort_bytes = 0, fft_work_bytes = 0, bytes = 0;
     rocfft_plan fwd = nullptr, bwd = nullptr;
     rocfft_execution_info info = nullptr;
     bool timing = false;
@@ -458,8 +438,9 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
                               !(d->flags & SE_PLAN_NO_PLANE_SPREAD));
     const size_t M = g.M, Mz = g.Mz, N = d->n_max, ns = n_sys;
     size_t b = 0;
-    b += 4 * N * 4;                                       // key, rank, members, positions
-    b += 2 * (size_t)(ns * tl.nbins + 1) * 4;             // count, row_start
+    b += 4 * N * 4;                                       // two (key, index) pairs of the key sort
+    b += 2 * 256 * 4 * (N / 1024 + 1) + (64u << 10);      // sort histograms and scan area (bound)
+    b += (size_t)(ns * tl.ntiles + 1) * 4;                // tile starts
     b += 4 * (size_t)(ns * tl.ntiles + 2) * 4;            // tile_count, nchunk, chunk_start
     b += 3 * (size_t)tl.max_items * 4;                    // items
     b += N * (es == 4 ? 32 : 48);                         // records
@@ -493,25 +474,27 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int
     const long nb = (long)ns * tl.nbins;
     const int nt = ns * tl.ntiles;
     pl->rec_ev(0);
-    launch_fill_i32(pl->count, nb + 1, 0, s);
+    // sort key and index of every particle, then a stable sort by key: the records of a
+    // key keep the particles' index order, so the sorted order does not depend on timing
     if (grp)
-        launch_bin_keys_group<T>(pl->d_sys, S, blocks, pl->g, tl, pl->key, pl->rank, pl->count, s);
+        launch_sort_keys_group<T>(pl->d_sys, S, blocks, pl->g, tl, pl->key, pl->idx, s);
     else
-        launch_bin_keys<T>(x, N, pl->g, tl, pl->key, pl->rank, pl->count, s);
+        launch_sort_keys<T>(x, N, pl->g, tl, pl->key, pl->idx, s);
+    const int bits = 32 - __builtin_clz((unsigned)std::max<long>(nb, 1));   // keys 0 .. nb (discard)
+    const int* skey = nullptr;
+    int* pos = nullptr;   // pos[n]: the sorted position of particle n (the inverse permutation)
+    stable_sort_pairs(pl->key, pl->idx, pl->key2, pl->idx2, N, bits, true, pl->sort_tmp, pl->sort_bytes, s, &skey,
+                      &pos);
+    launch_tile_starts(skey, N, nt, tl.K, pl->tile_start, s);
+    launch_tile_chunks(pl->tile_start, nt, tl.chunk, pl->tile_count, pl->nchunk, s);
     size_t tb = pl->cub_bytes;
-    SE_HIP(scan_counts(pl->cub_tmp, tb, pl->count, pl->row_start, (int)(nb + 1), s));
-    launch_tile_chunks(pl->row_start, nt, tl.K, tl.chunk, pl->tile_count, pl->nchunk, s);
-    tb = pl->cub_bytes;
     SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->nchunk, pl->chunk_start, nt + 1, s));
-    launch_items(pl->tile_count, pl->row_start, pl->chunk_start, nt, tl.K, tl.chunk, pl->item_tile, pl->item_beg,
+    launch_items(pl->tile_count, pl->tile_start, pl->chunk_start, nt, tl.chunk, pl->item_tile, pl->item_beg,
                  pl->item_end, pl->n_items, s);
-    // position of every particle in (key, index) order: the particle order inside a key
-    // does not depend on the order of the counter atomics
-    launch_stable_positions(pl->key, pl->rank, pl->row_start, N, (int)nb, pl->memb, pl->pos, s);
     if (grp)
-        launch_scatter_group<T>(pl->d_sys, S, blocks, pl->g, tl, pl->pos, (Rec<T>*)pl->rec, s);
+        launch_scatter_group<T>(pl->d_sys, S, blocks, pl->g, tl, pos, (Rec<T>*)pl->rec, s);
     else
-        launch_scatter<T>(x, q, N, pl->g, tl, pl->pos, (Rec<T>*)pl->rec, s);
+        launch_scatter<T>(x, q, N, pl->g, tl, pos, (Rec<T>*)pl->rec, s);
     pl->rec_ev(1);
     // work items: at most one partial item per non-empty tile plus one per full chunk
     Tiling tlc = tl;
@@ -520,7 +503,7 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int
     if constexpr (sizeof(T) == 4) {
         if (tl.pencil)
             done = launch_spread_pencil((const Rec<float>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end,
-                                        pl->n_items, pl->row_start, pl->g, tlc, (float*)pl->partials, s);
+                                        pl->n_items, pl->tile_start, pl->g, tlc, (float*)pl->partials, s);
     }
     if (!done)
         launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
@@ -743,11 +726,12 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         SE_REQUIRE(ns * tl.nbins < (1ul << 31) - 1 && ns * tl.ntiles * (size_t)tl.K < (1ul << 31) - 1, SE_ERR_ARG,
                    "too many systems for the sort keys of this geometry");
         pl->key = (int*)pl->alloc(N * 4);
-        pl->rank = (int*)pl->alloc(N * 4);
-        pl->memb = (int*)pl->alloc(N * 4);
-        pl->pos = (int*)pl->alloc(N * 4);
-        pl->count = (int*)pl->alloc((ns * tl.nbins + 1) * 4);
-        pl->row_start = (int*)pl->alloc((ns * tl.nbins + 1) * 4);
+        pl->idx = (int*)pl->alloc(N * 4);
+        pl->key2 = (int*)pl->alloc(N * 4);
+        pl->idx2 = (int*)pl->alloc(N * 4);
+        pl->tile_start = (int*)pl->alloc((ns * tl.ntiles + 1) * 4);
+        pl->sort_bytes = sort_tmp_bytes((int)N);
+        pl->sort_tmp = pl->alloc(pl->sort_bytes);
         pl->tile_count = (int*)pl->alloc((ns * tl.ntiles + 1) * 4);
         pl->nchunk = (int*)pl->alloc((ns * tl.ntiles + 1) * 4);
         pl->chunk_start = (int*)pl->alloc((ns * tl.ntiles + 2) * 4);
@@ -760,15 +744,9 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         pl->grid = pl->alloc(ns * M * M * Mz * pl->esize);
         pl->spec = pl->alloc(ns * M * M * (Mz / 2 + 1) * 2 * pl->esize);
         pl->gtab = (double*)pl->alloc((pl->g.per == 3 ? 2 * M + 2 * Mz + 1 : M * M * (Mz / 2 + 1)) * sizeof(double));
-        SE_HIP(scan_counts(nullptr, pl->cub_bytes, pl->count, pl->row_start, (int)(ns * tl.nbins + 1),
-                           pl->stream));
-        {
-            // one temporary area for the scans of the key counts and of the tile chunks
-            size_t cb = 0;
-            SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, cb, pl->nchunk, pl->chunk_start,
-                                                    (int)(ns * tl.ntiles + 1), pl->stream));
-            pl->cub_bytes = std::max(pl->cub_bytes, cb);
-        }
+        // temporary area of the scan of the tile chunks
+        SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, pl->cub_bytes, pl->nchunk, pl->chunk_start,
+                                                (int)(ns * tl.ntiles + 1), pl->stream));
         if (ns > 1) {
             pl->d_sys = (SysDesc*)pl->alloc(ns * sizeof(SysDesc));
             for (int i = 0; i < 4; ++i) {
@@ -1063,12 +1041,14 @@ int se_plan_info(se_plan* pl, int out[8])
         out[2] = pl->tl.block;
         out[3] = pl->tl.ntiles;
         SE_HIP(hipStreamSynchronize(pl->stream));
-        int ni = 0, bad = 0;
+        int ni = 0, kept = 0;
         SE_HIP(hipMemcpy(&ni, pl->n_items, sizeof(int), hipMemcpyDeviceToHost));
-        const int nb = pl->last_S * pl->tl.nbins;   // the discard bucket's key
-        SE_HIP(hipMemcpy(&bad, pl->count + count_slot(nb, nb + 1), sizeof(int), hipMemcpyDeviceToHost));
+        // the discard bucket sorts after every tile: its records start at the end entry
+        // of the tile starts
+        SE_HIP(hipMemcpy(&kept, pl->tile_start + (long)pl->last_S * pl->tl.ntiles, sizeof(int),
+                         hipMemcpyDeviceToHost));
         out[4] = ni;
-        out[5] = bad;
+        out[5] = pl->last_N > 0 ? pl->last_N - kept : 0;
         out[6] = pl->tl.S;
         out[7] = pl->tl.pencil;
     });

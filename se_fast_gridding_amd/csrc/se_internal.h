@@ -299,37 +299,32 @@ void stable_sort_pairs(int* k0, int* v0, int* k1, int* v1, int n, int bits, bool
 // if nslots > 0, else at k
 void launch_run_counts(const int* skey, int n, int nslots, int* count, hipStream_t s);
 
-// bucketing: key[n] and a provisional rank inside the key from the counter atomics, counts
-// per key; after the scan of the counts, launch_stable_positions gives each particle's
-// position pos[n] in (key, index) order (memb: N ints of scratch), where the scatter
-// writes record n
+// bucketing: key[n] (tile, pencil, row; the discard key nbins past every tile) and
+// idx[n] = n for the stable key sort; after the sort, the tile starts from the sorted keys
+// (nt + 1 entries; K keys per tile)
 template <typename T>
-void launch_bin_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key,
-                     int* rank, int* count, hipStream_t s);
-void launch_stable_positions(const int* key, const int* rank, const int* row_start, int N, int nb, int* memb,
-                             int* pos, hipStream_t s);
-void launch_tile_chunks(const int* row_start, int ntiles, int K, int chunk, int* tile_count,
-                        int* nchunk, hipStream_t s);
-void launch_items(const int* tile_count, const int* row_start, const int* chunk_start, int ntiles,
-                  int K, int chunk, int* item_tile, int* item_beg, int* item_end, int* n_items,
-                  hipStream_t s);
+void launch_sort_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key, int* idx, hipStream_t s);
+void launch_tile_starts(const int* skey, int N, int nt, int K, int* tile_start, hipStream_t s);
+void launch_tile_chunks(const int* tile_start, int ntiles, int chunk, int* tile_count, int* nchunk, hipStream_t s);
+void launch_items(const int* tile_count, const int* tile_start, const int* chunk_start, int ntiles, int chunk,
+                  int* item_tile, int* item_beg, int* item_end, int* n_items, hipStream_t s);
 template <typename T>
 void launch_scatter(const double* x, const double* q, int N, const Geom& g, const Tiling& tl,
                     const int* pos, Rec<T>* rec, hipStream_t s);
 // grouped forms: `blocks` = sum over systems of ceil(N/256); discard bucket S * nbins;
 // idx / pos are indexed by the flat particle index off + n
 template <typename T>
-void launch_bin_keys_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, int* key,
-                           int* rank, int* count, hipStream_t s);
+void launch_sort_keys_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, int* key,
+                            int* idx, hipStream_t s);
 template <typename T>
 void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, const int* pos,
                           Rec<T>* rec, hipStream_t s);
 // pencil kernels (se_pencil.hip): FP32 only; return false if P has no compiled pencil kernel
 bool pencil_supported(int P);
 // pencil spread; where tl.plane is set, whole sparse tiles take the plane spread
-// (row_start: the sort keys' first records)
+// (tile_start: the tiles' first records)
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
-                          const int* n_items, const int* row_start, const Geom& g, const Tiling& tl,
+                          const int* n_items, const int* tile_start, const Geom& g, const Tiling& tl,
                           float* partials, hipStream_t s);
 // sys != null: grouped solve, the outputs and N come from sys[tile / ntiles] and the
 // grids are stacked; phi/force then only select the output mode.  side (with the events

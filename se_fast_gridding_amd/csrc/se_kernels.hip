@@ -132,11 +132,10 @@ __device__ __forceinline__ int row_key(int w0, int w1, int w2, const Tiling& tl)
     return tile * tl.K + (((lx >> 2) * tl.TS + lz) * tl.TS + ly);
 }
 
-// Sort key of every particle and a provisional rank inside its key (the order of the
-// counter atomics); k_stable_rank later replaces the rank by the particle's index order.
+// Sort key of every particle and its index, the value the stable key sort carries.
 template <typename T>
-__global__ void k_bin_keys(const double* __restrict__ x, int N, Geom g, Tiling tl,
-                           int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count)
+__global__ void k_sort_keys(const double* __restrict__ x, int N, Geom g, Tiling tl, int* __restrict__ key,
+                            int* __restrict__ idx)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -147,117 +146,37 @@ __global__ void k_bin_keys(const double* __restrict__ x, int N, Geom g, Tiling t
     int w2 = window_start_z(x[n + 2l * N], g, t, ok2);
     // non-finite or absurd positions (and, for SE2P, positions outside the slab) go to a
     // discard bucket after all tiles
-    int k = (ok0 && ok1 && ok2) ? row_key(w0, w1, w2, tl) : tl.nbins;
-    key[n] = k;
-    rank[n] = atomicAdd(&count[count_slot(k, tl.nbins + 1)], 1);
+    key[n] = (ok0 && ok1 && ok2) ? row_key(w0, w1, w2, tl) : tl.nbins;
+    idx[n] = n;
 }
 
 template <typename T>
-void launch_bin_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key, int* rank, int* count,
-                     hipStream_t s)
+void launch_sort_keys(const double* x, int N, const Geom& g, const Tiling& tl, int* key, int* idx, hipStream_t s)
 {
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_bin_keys<T>, dim3(cdiv(N, 256)), dim3(256), 0, s, x, N, g, tl, key, rank, count);
+    hipLaunchKernelGGL(k_sort_keys<T>, dim3(cdiv(N, 256)), dim3(256), 0, s, x, N, g, tl, key, idx);
 }
 
-// Members of every key in the order of the counter atomics: memb[row_start[k] + rank[n]] = n.
-__global__ void k_members(const int* __restrict__ key, const int* __restrict__ rank, const int* __restrict__ row_start,
-                          int N, int* __restrict__ memb)
+// First sorted position of every tile, from the sorted keys: position i starts the tiles
+// (tile(i - 1), tile(i)] (tile = key / K; the discard key nt K counts as tile nt; the
+// sentinels are -1 before the first position and nt + 1 past the last), so each of the
+// nt + 1 entries is written exactly once, with no atomics and no scan.
+__global__ void k_tile_starts(const int* __restrict__ skey, int N, int nt, int K, int* __restrict__ tile_start)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n < N) memb[row_start[key[n]] + rank[n]] = n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > N) return;
+    const int prev = i > 0 ? skey[i - 1] / K : -1;
+    const int cur = i < N ? skey[i] / K : nt + 1;
+    for (int t = prev + 1; t <= min(cur, nt); ++t) tile_start[t] = i;
 }
 
-// Stable position of every particle: the key's first row plus the number of members of the
-// key with a smaller index, so the order inside a key is the index order whatever the
-// order of the counter atomics.  One thread per member position: the threads of one key
-// read the same members at the same time (a broadcast), and a key of s members costs
-// s steps per thread.  Keys of more than STABLE_RANK_MAX members (over 16k particles per
-// grid cell) keep the atomics' order, which bounds the cost at STABLE_RANK_MAX steps per
-// particle.
-// A work-group whose 256 positions all lie in one key (the dense keys of clustered input)
-// stages the key's members through LDS, 256 at a time, and every thread compares against
-// them by broadcast LDS reads.
-constexpr int STABLE_RANK_MAX = 1 << 16;
-constexpr int STABLE_RANK_THREADS = 256;
-__global__ __launch_bounds__(STABLE_RANK_THREADS) void k_stable_rank(const int* __restrict__ memb,
-                                                                     const int* __restrict__ key,
-                                                                     const int* __restrict__ row_start, int N, int nb,
-                                                                     int* __restrict__ pos)
+void launch_tile_starts(const int* skey, int N, int nt, int K, int* tile_start, hipStream_t s)
 {
-    __shared__ __align__(16) int stage[STABLE_RANK_THREADS];
-    __shared__ int bounds[2];
-    const int tid = threadIdx.x, i0 = blockIdx.x * STABLE_RANK_THREADS, i = i0 + tid;
-    int n = -1, a = 0, b = 0;
-    if (i < N) {
-        n = memb[i];
-        const int k = key[n];
-        a = row_start[k];
-        b = k < nb ? row_start[k + 1] : N;
-    }
-    // the group shares one key iff its first and last positions share one
-    const int il = min(i0 + STABLE_RANK_THREADS, N) - 1;
-    if (i == i0) bounds[0] = a;
-    if (i == il) bounds[1] = b;
-    __syncthreads();
-    const int A = bounds[0], B = bounds[1];   // block-uniform
-    const bool one_key = A == a && B == b && i0 >= a && il < b;
-    if (__syncthreads_and(i >= N || one_key) && B - A <= STABLE_RANK_MAX) {
-        // every thread (also those past N) runs the same staged loop over [A, B)
-        int r = 0;
-        for (int j0 = A; j0 < B; j0 += STABLE_RANK_THREADS) {
-            stage[tid] = j0 + tid < B ? memb[j0 + tid] : 0x7fffffff;
-            __syncthreads();
-            const int4* st4 = reinterpret_cast<const int4*>(stage);
-#pragma unroll 8
-            for (int t = 0; t < STABLE_RANK_THREADS / 4; ++t) {
-                const int4 v = st4[t];
-                r += (v.x < n) + (v.y < n) + (v.z < n) + (v.w < n);
-            }
-            __syncthreads();
-        }
-        if (i < N) pos[n] = A + r;
-        return;
-    }
-    if (i >= N) return;
-    // a full wave whose lanes all lie in one key reads the key's members 64 at a time, one
-    // per lane, and compares against each of them through a lane broadcast (a wave with
-    // lanes past N has lanes that returned above and hold no member: it takes the
-    // per-lane loop below)
-    const int wa = __builtin_amdgcn_readfirstlane(a), wbnd = __builtin_amdgcn_readfirstlane(b);
-    const bool wave_key = __ballot(1) == ~0ull && __ballot(a == wa && b == wbnd) == ~0ull;
-    if (wave_key && wbnd - wa <= STABLE_RANK_MAX) {
-        const int lane = threadIdx.x & 63;
-        int r = 0;
-        for (int j0 = wa; j0 < wbnd; j0 += 64) {
-            const int v = j0 + lane < wbnd ? memb[j0 + lane] : 0x7fffffff;
-#pragma unroll 16
-            for (int t = 0; t < 64; ++t) r += __builtin_amdgcn_readlane(v, t) < n;
-        }
-        pos[n] = wa + r;
-        return;
-    }
-    int r = i - a;
-    if (b - a <= STABLE_RANK_MAX) {
-        r = 0;
-        int j = a;
-        for (; j + 4 <= b; j += 4) r += (memb[j] < n) + (memb[j + 1] < n) + (memb[j + 2] < n) + (memb[j + 3] < n);
-        for (; j < b; ++j) r += memb[j] < n;
-    }
-    pos[n] = a + r;
+    hipLaunchKernelGGL(k_tile_starts, dim3(cdiv(N + 1, 256)), dim3(256), 0, s, skey, N, nt, K, tile_start);
 }
 
-void launch_stable_positions(const int* key, const int* rank, const int* row_start, int N, int nb, int* memb,
-                             int* pos, hipStream_t s)
-{
-    if (N <= 0) return;
-    hipLaunchKernelGGL(k_members, dim3(cdiv(N, 256)), dim3(256), 0, s, key, rank, row_start, N, memb);
-    hipLaunchKernelGGL(k_stable_rank, dim3(cdiv(N, STABLE_RANK_THREADS)), dim3(STABLE_RANK_THREADS), 0, s, memb, key,
-                       row_start, N, nb, pos);
-}
-
-// Per tile: particle count (from the row-key starts) and number of work items.
-__global__ void k_tile_chunks(const int* __restrict__ row_start, int ntiles, int K, int chunk,
+// Per tile: particle count (from the tile starts) and number of work items.
+__global__ void k_tile_chunks(const int* __restrict__ tile_start, int ntiles, int chunk,
                               int* __restrict__ tile_count, int* __restrict__ nchunk)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,27 +185,26 @@ __global__ void k_tile_chunks(const int* __restrict__ row_start, int ntiles, int
         nchunk[t] = 0;
         return;
     }
-    const int c = row_start[(long)(t + 1) * K] - row_start[(long)t * K];
+    const int c = tile_start[t + 1] - tile_start[t];
     tile_count[t] = c;
     nchunk[t] = (c + chunk - 1) / chunk;
 }
 
-void launch_tile_chunks(const int* row_start, int ntiles, int K, int chunk, int* tile_count, int* nchunk,
-                        hipStream_t s)
+void launch_tile_chunks(const int* tile_start, int ntiles, int chunk, int* tile_count, int* nchunk, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_tile_chunks, dim3(cdiv(ntiles + 1, 256)), dim3(256), 0, s, row_start, ntiles, K, chunk,
+    hipLaunchKernelGGL(k_tile_chunks, dim3(cdiv(ntiles + 1, 256)), dim3(256), 0, s, tile_start, ntiles, chunk,
                        tile_count, nchunk);
 }
 
-__global__ void k_items(const int* __restrict__ tile_count, const int* __restrict__ row_start,
-                        const int* __restrict__ chunk_start, int ntiles, int K, int chunk,
+__global__ void k_items(const int* __restrict__ tile_count, const int* __restrict__ tile_start,
+                        const int* __restrict__ chunk_start, int ntiles, int chunk,
                         int* __restrict__ item_tile, int* __restrict__ item_beg,
                         int* __restrict__ item_end, int* __restrict__ n_items)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) *n_items = chunk_start[ntiles];
     if (t >= ntiles) return;
-    const int b = row_start[(long)t * K], e = b + tile_count[t];
+    const int b = tile_start[t], e = b + tile_count[t];
     int it = chunk_start[t];
     for (int p = b; p < e; p += chunk, ++it) {
         item_tile[it] = t;
@@ -295,11 +213,11 @@ __global__ void k_items(const int* __restrict__ tile_count, const int* __restric
     }
 }
 
-void launch_items(const int* tile_count, const int* row_start, const int* chunk_start, int ntiles, int K,
-                  int chunk, int* item_tile, int* item_beg, int* item_end, int* n_items, hipStream_t s)
+void launch_items(const int* tile_count, const int* tile_start, const int* chunk_start, int ntiles, int chunk,
+                  int* item_tile, int* item_beg, int* item_end, int* n_items, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_items, dim3(cdiv(ntiles, 256)), dim3(256), 0, s, tile_count, row_start, chunk_start,
-                       ntiles, K, chunk, item_tile, item_beg, item_end, n_items);
+    hipLaunchKernelGGL(k_items, dim3(cdiv(ntiles, 256)), dim3(256), 0, s, tile_count, tile_start, chunk_start,
+                       ntiles, chunk, item_tile, item_beg, item_end, n_items);
 }
 
 // Records in sorted order: particle n goes to its sorted position pos[n].
@@ -346,8 +264,8 @@ __device__ __forceinline__ int block_system(const SysDesc* __restrict__ sys, int
 }
 
 template <typename T>
-__global__ void k_bin_keys_group(const SysDesc* __restrict__ sys, int S, Geom g, Tiling tl, int* __restrict__ key,
-                                 int* __restrict__ rank, int* __restrict__ count)
+__global__ void k_sort_keys_group(const SysDesc* __restrict__ sys, int S, Geom g, Tiling tl, int* __restrict__ key,
+                                  int* __restrict__ idx)
 {
     const int si = block_system(sys, S, blockIdx.x);
     const SysDesc d = sys[si];
@@ -358,17 +276,16 @@ __global__ void k_bin_keys_group(const SysDesc* __restrict__ sys, int S, Geom g,
     const int w0 = window_start(d.x[n], g, t, ok0);
     const int w1 = window_start(d.x[n + (long)d.N], g, t, ok1);
     const int w2 = window_start_z(d.x[n + 2l * d.N], g, t, ok2);
-    const int k = (ok0 && ok1 && ok2) ? si * tl.nbins + row_key(w0, w1, w2, tl) : S * tl.nbins;
-    key[d.off + n] = k;
-    rank[d.off + n] = atomicAdd(&count[count_slot(k, S * tl.nbins + 1)], 1);
+    key[d.off + n] = (ok0 && ok1 && ok2) ? si * tl.nbins + row_key(w0, w1, w2, tl) : S * tl.nbins;
+    idx[d.off + n] = d.off + n;
 }
 
 template <typename T>
-void launch_bin_keys_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, int* key,
-                           int* rank, int* count, hipStream_t s)
+void launch_sort_keys_group(const SysDesc* sys, int S, int blocks, const Geom& g, const Tiling& tl, int* key,
+                            int* idx, hipStream_t s)
 {
     if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_bin_keys_group<T>, dim3(blocks), dim3(256), 0, s, sys, S, g, tl, key, rank, count);
+    hipLaunchKernelGGL(k_sort_keys_group<T>, dim3(blocks), dim3(256), 0, s, sys, S, g, tl, key, idx);
 }
 
 template <typename T>
@@ -1434,12 +1351,11 @@ void launch_reorder_gather(const double* xin, const double* qin, const int* perm
 
 // ---- explicit instantiations ----------------------------------------------------
 #define SE_INST(T)                                                                                               \
-    template void launch_bin_keys<T>(const double*, int, const Geom&, const Tiling&, int*, int*, int*,          \
-                                     hipStream_t);                                                               \
+    template void launch_sort_keys<T>(const double*, int, const Geom&, const Tiling&, int*, int*, hipStream_t);  \
     template void launch_scatter<T>(const double*, const double*, int, const Geom&, const Tiling&, const int*,  \
                                     Rec<T>*, hipStream_t);                                                       \
-    template void launch_bin_keys_group<T>(const SysDesc*, int, int, const Geom&, const Tiling&, int*, int*,    \
-                                           int*, hipStream_t);                                                   \
+    template void launch_sort_keys_group<T>(const SysDesc*, int, int, const Geom&, const Tiling&, int*, int*,   \
+                                            hipStream_t);                                                        \
     template void launch_scatter_group<T>(const SysDesc*, int, int, const Geom&, const Tiling&, const int*,     \
                                           Rec<T>*, hipStream_t);                                                 \
     template void launch_spread_tiles<T>(const Rec<T>*, const int*, const int*, const int*, const int*,         \

@@ -216,17 +216,17 @@ struct GatherRing<P, P, R, FORCE> {
 
 }  // namespace
 
-// A sparse item: a whole tile (its range is every record of the tile's keys, so it was
-// not cut into chunks) of fewer than sparse_max particles.
+// A sparse item: a whole tile (its range is every record of the tile, so it was not cut
+// into chunks) of fewer than sparse_max particles.
 __device__ __forceinline__ bool sparse_item(const int* __restrict__ item_tile, const int* __restrict__ item_beg,
-                                            const int* __restrict__ item_end, const int* __restrict__ row_start,
-                                            const Tiling& tl, int item, int sparse_max)
+                                            const int* __restrict__ item_end, const int* __restrict__ tile_start,
+                                            int item, int sparse_max)
 {
     if (sparse_max <= 0) return false;
     const int beg = item_beg[item], end = item_end[item];
     if (end - beg >= sparse_max) return false;
-    const int kb = item_tile[item] * tl.K;
-    return beg == row_start[kb] && end == row_start[kb + tl.K];
+    const int t = item_tile[item];
+    return beg == tile_start[t] && end == tile_start[t + 1];
 }
 
 template <int P>
@@ -235,14 +235,14 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
                                                        const int* __restrict__ item_end,
                                                        const int* __restrict__ n_items, Geom g, Tiling tl,
                                                        float* __restrict__ partials, const int* __restrict__ item_tile,
-                                                       const int* __restrict__ row_start, int sparse_max)
+                                                       const int* __restrict__ tile_start, int sparse_max)
 {
     constexpr int R = ((P + 3) * P + 63) / 64;
     const int item = blockIdx.x;
     if (item >= *n_items) return;   // uniform across the work-group
     // sparse items (whole tiles of fewer than sparse_max particles) are the plane
     // spread's (sparse_max > 0)
-    if (sparse_item(item_tile, item_beg, item_end, row_start, tl, item, sparse_max)) return;
+    if (sparse_item(item_tile, item_beg, item_end, tile_start, item, sparse_max)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ float wmax[spread_pencil_max_threads(P) / 64];
     double* sub = reinterpret_cast<double*>(smem_raw);   // [y][x][z], S^3
@@ -433,14 +433,14 @@ __global__ __launch_bounds__(1024, 8) void k_spread_plane(const Rec<float>* __re
                                                           const int* __restrict__ item_beg,
                                                           const int* __restrict__ item_end,
                                                           const int* __restrict__ n_items,
-                                                          const int* __restrict__ row_start, Geom g, Tiling tl,
+                                                          const int* __restrict__ tile_start, Geom g, Tiling tl,
                                                           float* __restrict__ partials, int sparse_max)
 {
     constexpr int SM = 12 + P - 1;   // largest sub-grid side (tile side <= 12)
     static_assert(SM <= 32 && P <= 16, "the y table holds 32 rows, the x and z tables 16 + 16 + 16");
     const int item = blockIdx.x;
     if (item >= *n_items) return;
-    if (!sparse_item(item_tile, item_beg, item_end, row_start, tl, item, sparse_max)) return;
+    if (!sparse_item(item_tile, item_beg, item_end, tile_start, item, sparse_max)) return;
     __shared__ float tx[PLANE_BATCH][PLANE_TW], tz[PLANE_BATCH][PLANE_TW], ty[PLANE_BATCH][32];
     const int beg = item_beg[item], end = item_end[item];
     const int S = tl.S, tid = threadIdx.x, nthr = blockDim.x;
@@ -992,7 +992,7 @@ static void allow_lds_p(const void* fn, size_t bytes)
 int spread_sparse_max(const Tiling& tl) { return tl.plane ? mfma_dense_min(tl) : 0; }
 
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
-                          const int* n_items, const int* row_start, const Geom& g, const Tiling& tl, float* partials,
+                          const int* n_items, const int* tile_start, const Geom& g, const Tiling& tl, float* partials,
                           hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(double);
@@ -1004,10 +1004,10 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int
     do {                                                                                                     \
         allow_lds_p((const void*)k_spread_pencil<PV>, lds);                                                  \
         hipLaunchKernelGGL(k_spread_pencil<PV>, gr, bl, lds, s, rec, item_beg, item_end, n_items, g, tl,    \
-                           partials, item_tile, row_start, sparse_max);                                      \
+                           partials, item_tile, tile_start, sparse_max);                                      \
         if (sparse_max > 0)                                                                                  \
             hipLaunchKernelGGL(k_spread_plane<PV>, gr, dim3(64 * ((tl.S + 1) / 2)), 0, s, rec, item_tile,      \
-                               item_beg, item_end, n_items, row_start, g, tl, partials, sparse_max);         \
+                               item_beg, item_end, n_items, tile_start, g, tl, partials, sparse_max);         \
     } while (0)
     switch (g.P) {
     case 8: SE_SP(8); return true;

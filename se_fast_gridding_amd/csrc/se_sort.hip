@@ -1,15 +1,15 @@
-// Stable LSD radix sort of (key, value) int pairs, used by the bucketing of the real-space
-// cell lists: the order of equal keys is the input order, so the particle order inside a
-// cell does not depend on the timing of any atomic.  (The engine's bucketing keeps its
-// counting sort and fixes the order inside each key afterwards, which costs less at the
-// engine's key counts; see k_stable_rank.)
+// Stable LSD radix sort of (key, value) int pairs, used by the engine's bucketing (sort key
+// = tile, pencil, row) and by the real-space cell lists: the order of equal keys is the
+// input order, so the particle order inside a key does not depend on the timing of any
+// atomic.
 //
 // Each pass sorts by one 8-bit digit in three launches:
-//   k_sort_hist     per tile of 4096 items, the count of each digit (LDS counters)
+//   k_sort_hist     per tile of 1024 to 4096 items, the count of each digit (LDS counters)
 //   (scan)          exclusive scan of the digit-major table hist[digit * ntile + tile]
 //   k_sort_scatter  per tile, each item's position = the scanned base of (digit, tile) +
-//                   the number of earlier items of the tile with the same digit.
-// The scatter walks its tile in 16 rounds of 256 items; inside a round, a wave finds the
+//                   the number of earlier items of the tile with the same digit; the
+//                   tile is staged in LDS in that order and written out from there.
+// The scatter walks its tile in rounds of 256 items; inside a round, a wave finds the
 // lanes that share a lane's digit with 8 ballots (one per digit bit) and takes the lane's
 // rank among them from a popcount, the waves' per-digit counts are combined in LDS in wave
 // order, and a per-digit running count carries the rounds.  No atomics order anything and
@@ -25,20 +25,21 @@ namespace {
 
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
-constexpr int SORT_ROUNDS = 16;
-constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;   // items per tile
+constexpr int SORT_ROUNDS = 16;                         // most rounds of 256 items per tile
+constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;   // most items per tile
+constexpr int SORT_MIN_ROUNDS = 4;
 constexpr int SORT_RADIX = 256;
 
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const int* __restrict__ keys, int n, int shift, int ntile,
-                                                            int* __restrict__ hist)
+                                                            int rounds, int* __restrict__ hist)
 {
     __shared__ int cnt[SORT_RADIX];
     const int tid = threadIdx.x;
     cnt[tid] = 0;
     __syncthreads();
-    const long base = (long)blockIdx.x * SORT_TILE;
+    const long base = (long)blockIdx.x * rounds * SORT_THREADS;
 #pragma unroll 4
-    for (int r = 0; r < SORT_ROUNDS; ++r) {
+    for (int r = 0; r < rounds; ++r) {
         const long i = base + r * SORT_THREADS + tid;
         if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (SORT_RADIX - 1)], 1);
     }
@@ -47,22 +48,44 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const int* __restric
 }
 
 // kout / vout: sorted pairs.  inv != null (last pass): inv[v] = the item's sorted position
-// instead of vout.
+// instead of vout.  The tile's items are first placed in LDS in (digit, input) order --
+// each digit's items start at the digit's offset within the tile (an exclusive scan of
+// the tile's histogram) -- and then written out in LDS order, so that the items of one
+// digit, which are consecutive in the output, leave in consecutive lanes.
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(const int* __restrict__ kin, const int* __restrict__ vin,
-                                                               int n, int shift, int ntile,
+                                                               int n, int shift, int ntile, int rounds,
+                                                               const int* __restrict__ hist,
                                                                const int* __restrict__ offs, int* __restrict__ kout,
                                                                int* __restrict__ vout, int* __restrict__ inv)
 {
-    __shared__ int run[SORT_RADIX];               // next free position of each digit
+    __shared__ int lk[SORT_TILE], lv[SORT_TILE];  // the tile in (digit, input) order
+    __shared__ int lofs[SORT_RADIX];              // first LDS slot of each digit
+    __shared__ int gofs[SORT_RADIX];              // first output position of each digit
+    __shared__ int run[SORT_RADIX];               // items of each digit placed so far
     __shared__ int wcnt[SORT_WAVES][SORT_RADIX];  // this round's count of each digit per wave
+    __shared__ int wsum[SORT_WAVES];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    run[tid] = offs[(long)tid * ntile + blockIdx.x];
+    const int c = hist[(long)tid * ntile + blockIdx.x];
+    gofs[tid] = offs[(long)tid * ntile + blockIdx.x];
+    // exclusive scan of the tile's digit counts: wave prefix sums, then the waves' totals
+    int incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[w] = incl;
+    run[tid] = 0;
 #pragma unroll
     for (int v = 0; v < SORT_WAVES; ++v) wcnt[v][tid] = 0;
     __syncthreads();
+    int pre = 0;
+    for (int u = 0; u < w; ++u) pre += wsum[u];
+    lofs[tid] = pre + incl - c;
+    __syncthreads();
     const unsigned long long lt = (1ull << lane) - 1ull;   // lanes below this one
-    const long base = (long)blockIdx.x * SORT_TILE;
-    for (int r = 0; r < SORT_ROUNDS; ++r) {
+    const long base = (long)blockIdx.x * rounds * SORT_THREADS;
+    for (int r = 0; r < rounds; ++r) {
         const long i = base + r * SORT_THREADS + tid;
         const bool ok = i < n;
         const int k = ok ? kin[i] : 0;
@@ -79,13 +102,13 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(const int* __rest
         if (ok && below == 0) wcnt[w][d] = __popcll(m);   // the group's lowest lane
         __syncthreads();
         if (ok) {
-            int p = run[d] + below;
+            int p = run[d] + below;   // rank among the tile's items of digit d
             for (int u = 0; u < w; ++u) p += wcnt[u][d];
-            kout[p] = k;
+            lk[lofs[d] + p] = k;
             if (inv)
-                inv[v] = p;
+                inv[v] = gofs[d] + p;
             else
-                vout[p] = v;
+                lv[lofs[d] + p] = v;
         }
         __syncthreads();
         int add = 0;
@@ -97,15 +120,35 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(const int* __rest
         run[tid] += add;
         __syncthreads();
     }
+    const int cnt = (int)min((long)rounds * SORT_THREADS, (long)n - base);
+    for (int i = tid; i < cnt; i += SORT_THREADS) {
+        const int k = lk[i], d = (k >> shift) & (SORT_RADIX - 1);
+        const int p = gofs[d] + (i - lofs[d]);
+        kout[p] = k;
+        if (!inv) vout[p] = lv[i];
+    }
 }
 
 }  // namespace
 
-int sort_tiles(int n) { return (int)((n + (long)SORT_TILE - 1) / SORT_TILE); }
+// Rounds of 256 items per tile: the most (up to 16) that still give two tiles per CU of
+// a 256-CU device, so that a small sort is not one long work-group per CU.
+static int sort_rounds(int n)
+{
+    int r = SORT_ROUNDS;
+    while (r > SORT_MIN_ROUNDS && (n + (long)r * SORT_THREADS - 1) / ((long)r * SORT_THREADS) < 512) r /= 2;
+    return r;
+}
+
+static int sort_tiles(int n, int rounds)
+{
+    return (int)((n + (long)rounds * SORT_THREADS - 1) / ((long)rounds * SORT_THREADS));
+}
 
 size_t sort_tmp_bytes(int n_cap)
 {
-    const int nt = std::max(1, sort_tiles(std::max(n_cap, 1)));
+    // the table of the smallest tiles, which every n <= n_cap fits
+    const int nt = std::max(1, sort_tiles(std::max(n_cap, 1), SORT_MIN_ROUNDS));
     size_t b = 0;
     SE_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, (int*)nullptr, (int*)nullptr, SORT_RADIX * nt));
     const size_t hist = (size_t)2 * SORT_RADIX * nt * sizeof(int);
@@ -118,7 +161,7 @@ void stable_sort_pairs(int* k0, int* v0, int* k1, int* v1, int n, int bits, bool
     *keys_out = k0;
     *vals_out = v0;
     if (n <= 0) return;
-    const int nt = sort_tiles(n);
+    const int rounds = sort_rounds(n), nt = sort_tiles(n, rounds);
     const size_t hist_bytes = (size_t)2 * SORT_RADIX * nt * sizeof(int);
     const size_t hist_pad = (hist_bytes + 255) & ~(size_t)255;
     SE_REQUIRE(tmp_bytes >= hist_pad, SE_ERR_CAPACITY, "sort temporary area too small");
@@ -130,13 +173,14 @@ void stable_sort_pairs(int* k0, int* v0, int* k1, int* v1, int n, int bits, bool
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         const bool last = p == passes - 1;
-        hipLaunchKernelGGL(k_sort_hist, dim3(nt), dim3(SORT_THREADS), 0, s, ki, n, shift, nt, hist);
+        hipLaunchKernelGGL(k_sort_hist, dim3(nt), dim3(SORT_THREADS), 0, s, ki, n, shift, nt, rounds, hist);
         size_t sb = tmp_bytes - hist_pad;
         SE_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, sb, hist, offs, SORT_RADIX * nt, s));
         // the last pass of an inverse sort writes the inverse permutation into the values
         // output (its sorted values are not needed)
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(SORT_THREADS), 0, s, ki, vi, n, shift, nt, offs, ko, vo,
-                           (last && inverse) ? vo : nullptr);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(SORT_THREADS), 0, s, ki, vi, n, shift, nt, rounds, hist, offs,
+                           ko,
+                           vo, (last && inverse) ? vo : nullptr);
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
