@@ -42,21 +42,30 @@ HBM_PEAK_GBS = 8000.0
 #   mfma_util_pct = rocprofv3 MfmaUtil = 100 SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE SIMD_NUM)
 #   fma_insts     = SQ_INSTS_VALU_FMA_F32 (wave instructions, 64 lanes each) of all VALU
 #                   instructions SQ_INSTS_VALU
-# The gather of the clustered workload is two kernels (matrix-core gather on dense items,
-# vector gather on sparse items, concurrent on two streams); both are listed.
+# The spread and the gather of the clustered workload are two kernels each (spread: pencil
+# register ring on dense items, plane spread on sparse tiles, one after the other; gather:
+# matrix-core gather on dense items, vector gather on sparse items, concurrent on two
+# streams); each is listed under its stage.
 MEASURED = {
     ("clustered", 4_000_000, 16, 256, "fp32", 12): {
-        "k_spread_pencil": {"traffic_bytes": 1.0839e9, "valu_busy_pct": 53.0, "valu_insts": 7.331e8,
-                            "fma_insts": 3.286e8, "lds_bank_conflicts": 7.40e6, "lds_insts": 7.674e7},
-        "gather": {"traffic_bytes": 0.671e9 + 1.2406e9,
-                   "k_interp_mfma16": {"traffic_bytes": 0.671e9, "mfma_util_pct": 32.1, "valu_busy_pct": 36.1,
+        "spread": {"traffic_bytes": 251.4e6 + 831.8e6,
+                   "k_spread_pencil": {"traffic_bytes": 251.4e6, "valu_busy_pct": 47.9, "valu_insts": 4.062e8,
+                                       "fma_insts": 2.646e8, "lds_bank_conflicts": 1.13e6, "lds_insts": 2.758e7},
+                   "k_spread_plane": {"traffic_bytes": 831.8e6, "valu_busy_pct": 78.2, "valu_insts": 2.639e8,
+                                      "fma_insts": 1.784e8, "lds_bank_conflicts": 7.70e5, "lds_insts": 2.133e7}},
+        "gather": {"traffic_bytes": 670.6e6 + 1240.7e6,
+                   "k_interp_mfma16": {"traffic_bytes": 670.6e6, "mfma_util_pct": 32.4, "valu_busy_pct": 36.4,
                                        "mfma_insts": 1.633e8, "lds_bank_conflicts": 5.25e6, "lds_insts": 4.019e7},
-                   "k_interp_pencil": {"traffic_bytes": 1.2406e9, "valu_busy_pct": 62.4,
-                                       "lds_bank_conflicts": 6.356e7, "lds_insts": 3.752e7}},
+                   "k_interp_pencil": {"traffic_bytes": 1240.7e6, "valu_busy_pct": 63.2,
+                                       "lds_bank_conflicts": 6.36e6, "lds_insts": 3.997e7}},
     },
     ("uniform", 1_000_000, 16, 128, "fp32", 12): {
-        "k_spread_pencil": {"traffic_bytes": 137.3e6},
-        "gather": {"traffic_bytes": 273.7e6},
+        "spread": {"traffic_bytes": 137.3e6,
+                   "k_spread_pencil": {"traffic_bytes": 137.3e6, "valu_busy_pct": 64.9, "valu_insts": 1.57e8,
+                                       "fma_insts": 8.206e7, "lds_bank_conflicts": 8.19e5, "lds_insts": 1.514e7}},
+        "gather": {"traffic_bytes": 273.6e6,
+                   "k_interp_mfma16": {"traffic_bytes": 273.6e6, "mfma_util_pct": 32.7, "valu_busy_pct": 41.0,
+                                       "mfma_insts": 5.377e7, "lds_bank_conflicts": 1.406e7, "lds_insts": 1.94e7}},
     },
 }
 
@@ -280,7 +289,7 @@ def roofline(r, workload, prec_name):
     upd = float(N) * P ** 3
     kernels = {}
     meas_all = MEASURED.get((workload, N, P, M, prec_name, tile), {})
-    for name, key, fl, kms in (("k_spread_pencil", "k_spread_pencil", 2.0 * upd, ms["spread_kernel"]),
+    for name, key, fl, kms in (("spread (k_spread_pencil + k_spread_plane)", "spread", 2.0 * upd, ms["spread_kernel"]),
                                ("gather (k_interp_mfma16 + k_interp_pencil)", "gather", 4.0 * upd,
                                 ms["gather_kernel"])):
         meas = meas_all.get(key, {})
