@@ -980,6 +980,128 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Quad gather for sparse work items at P = 16 (items of fewer particles than the
+// matrix-core gather's threshold).  In a sparse tile a pencil holds one or two
+// particles, so the vector gather above loads all P ring rows (P R LDS reads and their
+// addresses), fetches the x and z factors across lanes and reduces four sums over the 64
+// lanes for each particle.  Here four lanes take one particle: lane q of the quad owns
+// the window's z points 4q .. 4q + 3 and walks the whole 16 x 16 (x, y) window, reading
+// the tile's LDS sub-grid at offsets fixed at compile time (the sub-grid side SS is a
+// template parameter), and contracts z, then y, then x in registers:
+//   a = sum_z G Fz,  b = sum_z G Fz'            (z, 4 points per lane)
+//   c0 += Fy a,  c1 += Fy' a,  c2 += Fy b        (y)
+//   phi += Fx c0,  Fx += Fx' c0,  Fy += Fx c1,  Fz += Fx c2   (x)
+// with F' = -c (t - o h) F the derivative factors (d folded into Fx), and a two-step
+// DPP sum over the quad at the end.  A particle costs 11 FMAs and 4 LDS reads per (x, y)
+// and lane: 176 FMA and 64 LDS wave instructions per particle, against about 500 VALU
+// and 80 LDS instructions in the vector gather.  Every sum has a fixed order.
+// ---------------------------------------------------------------------------------
+template <int SS, bool POT, bool FORCE>
+__global__ __launch_bounds__(512, 4) void k_interp_quad(const Rec<float>* __restrict__ rec,
+                                                     const int* __restrict__ item_tile,
+                                                     const int* __restrict__ item_beg,
+                                                     const int* __restrict__ item_end,
+                                                     const int* __restrict__ n_items,
+                                                     const float* __restrict__ grid, Geom g, Tiling tl, int N,
+                                                     double* __restrict__ phi, double* __restrict__ force,
+                                                     const SysDesc* __restrict__ sys, int dense_min)
+{
+    constexpr int P = 16, S2 = SS * SS;
+    const int item = blockIdx.x;
+    if (item >= *n_items) return;
+    const int beg = item_beg[item], end = item_end[item], n = end - beg;
+    // items of dense_min particles or more are the MFMA gather's
+    if (n >= dense_min) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], SS^3
+    const int M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    {
+        int tile = item_tile[item];
+        if (sys) {   // grouped solve: the item's system, its outputs and its grid
+            const int si = tile / tl.ntiles;
+            tile -= si * tl.ntiles;
+            grid += (long)si * M * M * Mz;
+            phi = sys[si].phi;
+            force = sys[si].force;
+            N = sys[si].N;
+        }
+        const int ox = (tile / (NT * NTz)) * tl.TS, oy = ((tile / NTz) % NT) * tl.TS, oz = (tile % NTz) * tl.TS;
+        load_tile_dma<true>(sub, grid, SS, M, Mz, ox, oy, oz, tid, nthr);
+    }
+    __syncthreads();
+
+    const int lane = tid & 63, q = lane & 3, wave = tid >> 6, nw = nthr >> 6;
+    const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
+    const float kc = -c * 1.4426950408889634f;
+    const double h3 = g.h * g.h * g.h;
+    for (int p0 = beg + 16 * wave; p0 < end; p0 += 16 * nw) {   // 16 particles per wave
+        const int p = p0 + (lane >> 2);
+        const bool valid = p < end;
+        const int pi = valid ? p : p0;
+        const float4 r = *reinterpret_cast<const float4*>(rec + pi);   // t[0], t[1], t[2], q
+        const int lo = rec[pi].lo, orig = rec[pi].orig;
+        const int lx = lo & 255, ly = (lo >> 8) & 255, lz = lo >> 16;
+        float fx[P], gx[P], fy[P], gy[P], fz[4], gz[4];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const float o = (float)(g.first + j) * h;
+            const float ax = r.x - o, ay = r.y - o;
+            fx[j] = gauss(kc, ax) * dn;
+            gx[j] = fx[j] * (-c * ax);
+            fy[j] = gauss(kc, ay);
+            gy[j] = fy[j] * (-c * ay);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float az = r.z - (float)(g.first + 4 * q + k) * h;
+            fz[k] = gauss(kc, az);
+            gz[k] = fz[k] * (-c * az);
+        }
+        const float* base = sub + (ly * SS + lx) * SS + lz + 4 * q;
+        float sp = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+        for (int x = 0; x < P; ++x) {
+            float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll
+            for (int y = 0; y < P; ++y) {
+                const float* row = base + y * S2 + x * SS;
+                const float G0 = row[0], G1 = row[1], G2 = row[2], G3 = row[3];
+                const float a = fmaf(G3, fz[3], fmaf(G2, fz[2], fmaf(G1, fz[1], G0 * fz[0])));
+                c0 = fmaf(fy[y], a, c0);
+                if (FORCE) {
+                    const float b = fmaf(G3, gz[3], fmaf(G2, gz[2], fmaf(G1, gz[1], G0 * gz[0])));
+                    c1 = fmaf(gy[y], a, c1);
+                    c2 = fmaf(fy[y], b, c2);
+                }
+            }
+            sp = fmaf(fx[x], c0, sp);
+            if (FORCE) {
+                sx = fmaf(gx[x], c0, sx);
+                sy = fmaf(fx[x], c1, sy);
+                sz = fmaf(fx[x], c2, sz);
+            }
+        }
+        // sums over the quad (quad_perm [1,0,3,2], then [2,3,0,1]): every lane holds them
+        sp = dpp_add<0x4E, 0xF>(dpp_add<0xB1, 0xF>(sp));
+        if (FORCE) {
+            sx = dpp_add<0x4E, 0xF>(dpp_add<0xB1, 0xF>(sx));
+            sy = dpp_add<0x4E, 0xF>(dpp_add<0xB1, 0xF>(sy));
+            sz = dpp_add<0x4E, 0xF>(dpp_add<0xB1, 0xF>(sz));
+        }
+        if (valid && q == 0) {
+            if (POT) store_out(phi, orig, h3 * (double)sp);
+            if (FORCE) {
+                const double qh = h3 * (double)r.w;
+                store_out(force, orig, qh * (double)sx);
+                store_out(force, orig + (long)N, qh * (double)sy);
+                store_out(force, orig + 2l * N, qh * (double)sz);
+            }
+        }
+    }
+}
+
 bool pencil_supported(int P) { return P == 8 || P == 11 || P == 16; }
 
 static void allow_lds_p(const void* fn, size_t bytes)
@@ -1063,6 +1185,39 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
 #undef SE_GP
 }
 
+// The quad gather of the sparse items (P = 16, sub-grid sides 19, 23 and 27: tile sides
+// 4, 8 and 12); false if the sub-grid side has no instance.
+static bool interp_quad_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
+                              const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
+                              double* phi, double* force, const SysDesc* sys, int dense_min, hipStream_t s)
+{
+    const size_t lds = ((size_t)tl.S * tl.S * tl.S + LDS_TILE_PAD) * sizeof(float);
+    dim3 gr(tl.max_items), bl(512);
+#define SE_GQ(SV, POT, FORCE)                                                                                 \
+    do {                                                                                                      \
+        allow_lds_p((const void*)k_interp_quad<SV, POT, FORCE>, lds);                                         \
+        hipLaunchKernelGGL((k_interp_quad<SV, POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg,         \
+                           item_end, n_items, grid, g, tl, N, phi, force, sys, dense_min);                    \
+    } while (0)
+#define SE_GQS(SV)                        \
+    do {                                  \
+        if (phi && force)                 \
+            SE_GQ(SV, true, true);        \
+        else if (force)                   \
+            SE_GQ(SV, false, true);       \
+        else                              \
+            SE_GQ(SV, true, false);       \
+    } while (0)
+    switch (tl.S) {
+    case 19: SE_GQS(19); return true;
+    case 23: SE_GQS(23); return true;
+    case 27: SE_GQS(27); return true;
+    default: return false;
+    }
+#undef SE_GQS
+#undef SE_GQ
+}
+
 bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
                           double* phi, double* force, const SysDesc* sys, hipStream_t s, hipStream_t side,
@@ -1091,8 +1246,10 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
                 SE_HIP(hipStreamWaitEvent(side, fork, 0));
                 sv = side;
             }
-            interp_pencil_modes<16>(rec, item_tile, item_beg, item_end, n_items, grid, g, tv, N, phi, force, sys,
-                                    dense_min, sv);
+            if (!interp_quad_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
+                                   dense_min, sv))
+                interp_pencil_modes<16>(rec, item_tile, item_beg, item_end, n_items, grid, g, tv, N, phi, force, sys,
+                                        dense_min, sv);
             if (side) SE_HIP(hipEventRecord(join, side));
             interp_mfma16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
                                 dense_min, s);

@@ -258,6 +258,45 @@ def test_mfma_gather_matches_vector_gather(oracle, clustered):
         assert rms_rel(a[0], ref_phi) < FP32_TOL and rms_rel(a[1], ref_F) < FP32_TOL_F, kw
 
 
+@pytest.mark.parametrize("clustered", [False, True])
+def test_quad_gather_of_sparse_items_matches_vector_gather(oracle, clustered):
+    """Sparse work items at P = 16 (fewer particles than the matrix-core threshold) take
+    the quad gather (four lanes per particle); at every tile side it has an instance for
+    (sub-grid sides 27, 23, 19) it agrees with the vector register-ring gather (the
+    SE_PLAN_NO_MFMA path) and with the FP64 oracle, for each output mode, and repeats bit
+    for bit."""
+    N, M, P, xi = 3000, 48, 16, 12.0
+    x, q = system(N, 310 + int(clustered), clustered=clustered)
+    m = 0.95 * np.sqrt(np.pi * P)
+    ref_phi = oracle.se3p(x, q, 1.0, M, P, m, xi)
+    ref_F = oracle.se3p(x, q, 1.0, M, P, m, xi, force=True)
+    xt, qt = torch.tensor(x, device=DEV), torch.tensor(q, device=DEV)
+    for kw in (dict(), dict(tile=8), dict(tile=4), dict(chunk=40)):
+        out = {}
+        for mf in (True, False, True):
+            plan = se.Plan(M, P, 1.0, xi, N, precision=se.PREC_FP32, mfma=mf, **kw)
+            phi = torch.zeros(N, dtype=torch.float64, device=DEV)
+            F = torch.zeros(3 * N, dtype=torch.float64, device=DEV)
+            p1 = torch.zeros(N, dtype=torch.float64, device=DEV)
+            f1 = torch.zeros(3 * N, dtype=torch.float64, device=DEV)
+            plan.solve(xt, qt, phi, F)
+            plan.interp(p1, None)
+            plan.interp(None, f1)
+            torch.cuda.synchronize()
+            assert plan.info()["pencil"] == 1
+            plan.close()
+            res = [t.cpu().numpy() for t in (phi, F, p1, f1)]
+            if mf in out:
+                assert all(np.array_equal(u, v) for u, v in zip(out[mf], res)), kw
+            out[mf] = res
+        a, b = out[True], out[False]
+        print("quad vs vector", kw, [rms_rel(u, v) for u, v in zip(a, b)], flush=True)
+        for u, v in zip(a, b):
+            assert rms_rel(u, v) < 2e-6, kw
+        assert rms_rel(a[2], a[0]) < 1e-7 and rms_rel(a[3], a[1]) < 1e-7, kw
+        assert rms_rel(a[0], ref_phi) < FP32_TOL and rms_rel(a[1], ref_F) < FP32_TOL_F, kw
+
+
 def clustered_system(N, seed=3, n_clusters=64, frac=0.8, sigma=0.01):
     """80 % of N in 64 Gaussian clusters (sigma = 0.01 L), 20 % uniform, q = +-1, zero net charge."""
     rng = np.random.default_rng(seed)
