@@ -487,8 +487,7 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int
                       &pos);
     launch_tile_starts(skey, N, nt, tl.K, pl->tile_start, s);
     launch_tile_chunks(pl->tile_start, nt, tl.chunk, pl->tile_count, pl->nchunk, s);
-    size_t tb = pl->cub_bytes;
-    SE_HIP(hipcub::DeviceScan::ExclusiveSum(pl->cub_tmp, tb, pl->nchunk, pl->chunk_start, nt + 1, s));
+    exclusive_scan(pl->nchunk, pl->chunk_start, nt + 1, pl->cub_tmp, pl->cub_bytes, s);
     launch_items(pl->tile_count, pl->tile_start, pl->chunk_start, nt, tl.chunk, pl->item_tile, pl->item_beg,
                  pl->item_end, pl->n_items, s);
     if (grp)

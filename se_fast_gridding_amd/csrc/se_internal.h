@@ -292,6 +292,10 @@ void launch_fill_bytes0(void* p, size_t bytes, hipStream_t s);
 // instead, (*vals_out)[v] = sorted position of the pair with value v (values must be a
 // permutation of [0, n)).  tmp holds sort_tmp_bytes(n) bytes or more.  Launches only
 // kernels (no memset), so it can be graph-captured.
+// Exclusive scan of n ints: one work-group for n <= SCAN_SMALL_MAX, else the library scan
+// with its temporary area tmp (tmp_bytes, sized by hipcub for n).
+constexpr int SCAN_SMALL_MAX = 32768;
+void exclusive_scan(const int* in, int* out, int n, void* tmp, size_t tmp_bytes, hipStream_t s);
 size_t sort_tmp_bytes(int n_cap);
 void stable_sort_pairs(int* k0, int* v0, int* k1, int* v1, int n, int bits, bool inverse, void* tmp,
                        size_t tmp_bytes, hipStream_t s, const int** keys_out, int** vals_out);

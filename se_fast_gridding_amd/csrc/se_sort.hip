@@ -129,6 +129,37 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(const int* __rest
     }
 }
 
+// Exclusive scan of n <= SCAN_SMALL_MAX ints by one work-group: each thread sums a run of
+// at most 32 consecutive entries, the run totals are scanned across the work-group (wave
+// prefix sums, then the waves' totals), and each thread writes its run.  One launch where
+// the library scan takes two, for the short tables of small solves.
+constexpr int SCAN_SMALL_THREADS = 1024;
+__global__ __launch_bounds__(SCAN_SMALL_THREADS) void k_scan_small(const int* __restrict__ in, int* __restrict__ out,
+                                                                   int n)
+{
+    __shared__ int wsum[SCAN_SMALL_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int per = (n + SCAN_SMALL_THREADS - 1) / SCAN_SMALL_THREADS;
+    const int b = min(tid * per, n), e = min(b + per, n);
+    int t = 0;
+    for (int i = b; i < e; ++i) t += in[i];
+    int incl = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int run = incl - t;
+    for (int u = 0; u < w; ++u) run += wsum[u];
+    for (int i = b; i < e; ++i) {
+        const int v = in[i];
+        out[i] = run;
+        run += v;
+    }
+}
+
 }  // namespace
 
 // Rounds of 256 items per tile: the most (up to 16) that still give two tiles per CU of
@@ -155,6 +186,17 @@ size_t sort_tmp_bytes(int n_cap)
     return ((hist + 255) & ~(size_t)255) + b;
 }
 
+void exclusive_scan(const int* in, int* out, int n, void* tmp, size_t tmp_bytes, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (n <= SCAN_SMALL_MAX) {
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SCAN_SMALL_THREADS), 0, s, in, out, n);
+        return;
+    }
+    size_t b = tmp_bytes;
+    SE_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, b, in, out, n, s));
+}
+
 void stable_sort_pairs(int* k0, int* v0, int* k1, int* v1, int n, int bits, bool inverse, void* tmp,
                        size_t tmp_bytes, hipStream_t s, const int** keys_out, int** vals_out)
 {
@@ -174,8 +216,7 @@ void stable_sort_pairs(int* k0, int* v0, int* k1, int* v1, int n, int bits, bool
         const int shift = 8 * p;
         const bool last = p == passes - 1;
         hipLaunchKernelGGL(k_sort_hist, dim3(nt), dim3(SORT_THREADS), 0, s, ki, n, shift, nt, rounds, hist);
-        size_t sb = tmp_bytes - hist_pad;
-        SE_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, sb, hist, offs, SORT_RADIX * nt, s));
+        exclusive_scan(hist, offs, SORT_RADIX * nt, scan_tmp, tmp_bytes - hist_pad, s);
         // the last pass of an inverse sort writes the inverse permutation into the values
         // output (its sorted values are not needed)
         hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(SORT_THREADS), 0, s, ki, vi, n, shift, nt, rounds, hist, offs,
