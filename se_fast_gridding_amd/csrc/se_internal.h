@@ -233,9 +233,14 @@ __device__ __forceinline__ float item_qmax_read(const float* wmax, int nw)
 // (3 waves per SIMD) at 768 threads; the smaller supports fit 128 at 1024.
 constexpr int spread_pencil_max_threads(int P) { return P == 16 ? 768 : 1024; }
 
-// The matrix-core gather takes the items with at least this many particles (about one
-// per pencil row, which a 4-particle group needs); the vector gather takes the others.
-inline int mfma_dense_min(const Tiling& tl) { return tl.K / 2 > 1 ? tl.K / 2 : 1; }
+// A sparse work item has fewer particles than half the tile's (pencil, row) keys: the
+// plane spread takes those whole tiles, and where the P = 16 row gather has no instance,
+// the vector gather takes those items instead of the matrix-core gather (a 4-particle
+// MFMA group needs about one particle per pencil row).
+inline int sparse_tile_max(const Tiling& tl) { return tl.K / 2 > 1 ? tl.K / 2 : 1; }
+// P = 16 FP32 gather split: items with fewer particles than this take the row gather,
+// the others the matrix-core gather (se_pencil.hip)
+int gather_dense_min(const Tiling& tl);
 
 // Particle record written by the bucketing scatter, read by spread and gather.
 template <typename T>

@@ -981,31 +981,32 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
 }
 
 // ---------------------------------------------------------------------------------
-// Quad gather for sparse work items at P = 16 (items of fewer particles than the
+// Row gather for sparse work items at P = 16 (items of fewer particles than the
 // matrix-core gather's threshold).  In a sparse tile a pencil holds one or two
 // particles, so the vector gather above loads all P ring rows (P R LDS reads and their
 // addresses), fetches the x and z factors across lanes and reduces four sums over the 64
-// lanes for each particle.  Here four lanes take one particle: lane q of the quad owns
-// the window's z points 4q .. 4q + 3 and walks the whole 16 x 16 (x, y) window, reading
-// the tile's LDS sub-grid at offsets fixed at compile time (the sub-grid side SS is a
-// template parameter), and contracts z, then y, then x in registers:
-//   a = sum_z G Fz,  b = sum_z G Fz'            (z, 4 points per lane)
-//   c0 += Fy a,  c1 += Fy' a,  c2 += Fy b        (y)
-//   phi += Fx c0,  Fx += Fx' c0,  Fy += Fx c1,  Fz += Fx c2   (x)
-// with F' = -c (t - o h) F the derivative factors (d folded into Fx), and a two-step
-// DPP sum over the quad at the end.  A particle costs 11 FMAs and 4 LDS reads per (x, y)
-// and lane: 176 FMA and 64 LDS wave instructions per particle, against about 500 VALU
-// and 80 LDS instructions in the vector gather.  Every sum has a fixed order.
+// lanes for each particle.  Here one 16-lane DPP row takes one particle: lane z of the row
+// owns the window's z point z and walks the whole 16 x 16 (x, y) window, reading the
+// tile's LDS sub-grid at offsets fixed at compile time (the sub-grid side SS is a template
+// parameter), and sums in registers
+//   c0 = sum_y Fy G,  c1 = sum_y Fy' G                  (2 FMAs per (x, y))
+//   u = sum_x Fx c0,  ux = sum_x Fx' c0,  uy = sum_x Fx c1
+// with F' = -c (t - o h) F the derivative factors (d folded into Fx).  The z contraction
+// comes last: phi = sum_z Fz u, F = sum_z (Fz ux, Fz uy, Fz' u), a four-step DPP sum over
+// the row.  Each step of a half-wave reads two runs of 16 consecutive LDS words (two
+// particles), so bank conflicts stay at most two-way.  A particle costs 140 FMA and 64 LDS
+// wave instructions, against about 500 VALU and 80 LDS instructions in the vector gather.
+// Every sum has a fixed order.
 // ---------------------------------------------------------------------------------
 template <int SS, bool POT, bool FORCE>
-__global__ __launch_bounds__(512, 4) void k_interp_quad(const Rec<float>* __restrict__ rec,
-                                                     const int* __restrict__ item_tile,
-                                                     const int* __restrict__ item_beg,
-                                                     const int* __restrict__ item_end,
-                                                     const int* __restrict__ n_items,
-                                                     const float* __restrict__ grid, Geom g, Tiling tl, int N,
-                                                     double* __restrict__ phi, double* __restrict__ force,
-                                                     const SysDesc* __restrict__ sys, int dense_min)
+__global__ __launch_bounds__(512, 4) void k_interp_row16(const Rec<float>* __restrict__ rec,
+                                                         const int* __restrict__ item_tile,
+                                                         const int* __restrict__ item_beg,
+                                                         const int* __restrict__ item_end,
+                                                         const int* __restrict__ n_items,
+                                                         const float* __restrict__ grid, Geom g, Tiling tl, int N,
+                                                         double* __restrict__ phi, double* __restrict__ force,
+                                                         const SysDesc* __restrict__ sys, int dense_min)
 {
     constexpr int P = 16, S2 = SS * SS;
     const int item = blockIdx.x;
@@ -1032,71 +1033,69 @@ __global__ __launch_bounds__(512, 4) void k_interp_quad(const Rec<float>* __rest
     }
     __syncthreads();
 
-    const int lane = tid & 63, q = lane & 3, wave = tid >> 6, nw = nthr >> 6;
+    const int lane = tid & 63, z = lane & 15, wave = tid >> 6, nw = nthr >> 6;
     const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
     const float kc = -c * 1.4426950408889634f;
     const double h3 = g.h * g.h * g.h;
-    for (int p0 = beg + 16 * wave; p0 < end; p0 += 16 * nw) {   // 16 particles per wave
-        const int p = p0 + (lane >> 2);
+    for (int p0 = beg + 4 * wave; p0 < end; p0 += 4 * nw) {   // 4 particles per wave
+        const int p = p0 + (lane >> 4);
         const bool valid = p < end;
         const int pi = valid ? p : p0;
         const float4 r = *reinterpret_cast<const float4*>(rec + pi);   // t[0], t[1], t[2], q
         const int lo = rec[pi].lo, orig = rec[pi].orig;
         const int lx = lo & 255, ly = (lo >> 8) & 255, lz = lo >> 16;
-        float fx[P], gx[P], fy[P], gy[P], fz[4], gz[4];
+        float fx[P], gx[P], fy[P], gy[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             const float o = (float)(g.first + j) * h;
             const float ax = r.x - o, ay = r.y - o;
             fx[j] = gauss(kc, ax) * dn;
-            gx[j] = fx[j] * (-c * ax);
             fy[j] = gauss(kc, ay);
-            gy[j] = fy[j] * (-c * ay);
+            if (FORCE) {
+                gx[j] = fx[j] * (-c * ax);
+                gy[j] = fy[j] * (-c * ay);
+            }
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float az = r.z - (float)(g.first + 4 * q + k) * h;
-            fz[k] = gauss(kc, az);
-            gz[k] = fz[k] * (-c * az);
-        }
-        const float* base = sub + (ly * SS + lx) * SS + lz + 4 * q;
-        float sp = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+        const float* base = sub + (ly * SS + lx) * SS + lz + z;
+        float u = 0.f, ux = 0.f, uy = 0.f;
 #pragma unroll
         for (int x = 0; x < P; ++x) {
-            float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+            float c0 = 0.f, c1 = 0.f;
 #pragma unroll
             for (int y = 0; y < P; ++y) {
-                const float* row = base + y * S2 + x * SS;
-                const float G0 = row[0], G1 = row[1], G2 = row[2], G3 = row[3];
-                const float a = fmaf(G3, fz[3], fmaf(G2, fz[2], fmaf(G1, fz[1], G0 * fz[0])));
-                c0 = fmaf(fy[y], a, c0);
-                if (FORCE) {
-                    const float b = fmaf(G3, gz[3], fmaf(G2, gz[2], fmaf(G1, gz[1], G0 * gz[0])));
-                    c1 = fmaf(gy[y], a, c1);
-                    c2 = fmaf(fy[y], b, c2);
-                }
+                const float G = base[y * S2 + x * SS];
+                c0 = fmaf(fy[y], G, c0);
+                if (FORCE) c1 = fmaf(gy[y], G, c1);
             }
-            sp = fmaf(fx[x], c0, sp);
+            u = fmaf(fx[x], c0, u);
             if (FORCE) {
-                sx = fmaf(gx[x], c0, sx);
-                sy = fmaf(fx[x], c1, sy);
-                sz = fmaf(fx[x], c2, sz);
+                ux = fmaf(gx[x], c0, ux);
+                uy = fmaf(fx[x], c1, uy);
             }
         }
-        // sums over the quad (quad_perm [1,0,3,2], then [2,3,0,1]): every lane holds them
-        sp = dpp_add<0x4E, 0xF>(dpp_add<0xB1, 0xF>(sp));
+        const float az = r.z - (float)(g.first + z) * h;
+        const float fz = gauss(kc, az);
+        // sums over the row: quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror
+        auto row_sum = [](float v) {
+            v = dpp_add<0xB1, 0xF>(v);
+            v = dpp_add<0x4E, 0xF>(v);
+            v = dpp_add<0x141, 0xF>(v);
+            return dpp_add<0x140, 0xF>(v);
+        };
+        const float sp = row_sum(fz * u);
+        float sx = 0.f, sy = 0.f, szz = 0.f;
         if (FORCE) {
-            sx = dpp_add<0x4E, 0xF>(dpp_add<0xB1, 0xF>(sx));
-            sy = dpp_add<0x4E, 0xF>(dpp_add<0xB1, 0xF>(sy));
-            sz = dpp_add<0x4E, 0xF>(dpp_add<0xB1, 0xF>(sz));
+            sx = row_sum(fz * ux);
+            sy = row_sum(fz * uy);
+            szz = row_sum(fz * (-c * az) * u);
         }
-        if (valid && q == 0) {
+        if (valid && z == 0) {
             if (POT) store_out(phi, orig, h3 * (double)sp);
             if (FORCE) {
                 const double qh = h3 * (double)r.w;
                 store_out(force, orig, qh * (double)sx);
                 store_out(force, orig + (long)N, qh * (double)sy);
-                store_out(force, orig + 2l * N, qh * (double)sz);
+                store_out(force, orig + 2l * N, qh * (double)szz);
             }
         }
     }
@@ -1111,7 +1110,7 @@ static void allow_lds_p(const void* fn, size_t bytes)
 
 // Items of fewer particles than this (whole sparse tiles) take the plane spread, the others
 // the pencil spread; 0 turns the plane spread off.
-int spread_sparse_max(const Tiling& tl) { return tl.plane ? mfma_dense_min(tl) : 0; }
+int spread_sparse_max(const Tiling& tl) { return tl.plane ? sparse_tile_max(tl) : 0; }
 
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const int* tile_start, const Geom& g, const Tiling& tl, float* partials,
@@ -1185,9 +1184,24 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
 #undef SE_GP
 }
 
-// The quad gather of the sparse items (P = 16, sub-grid sides 19, 23 and 27: tile sides
+static bool row16_supported(const Tiling& tl) { return tl.S == 19 || tl.S == 23 || tl.S == 27; }
+
+// Items below 5K/2 particles (2.5 per (pencil, row) key: 1080 at tile 12) take the row
+// gather, the others the matrix-core gather.  The row gather's cost per particle does not
+// depend on density; the MFMA gather's falls as its 4-particle groups fill.  Measured at
+// tile 12 (gather ms; uniform N = 1M at M = 128 with 824 particles per interior tile,
+// clustered N = 4M at M = 256, uniform N = 4M at M = 256):
+//   K: 0.60, 2.13, 2.13;  2K: 0.68, 2.08, 1.92;  5K/2: 0.50, 2.15, 1.95;  3K: 0.50, 2.17, 1.95
+// (2K sends the Poisson tail of the uniform 1M tiles to a nearly idle MFMA launch).
+int gather_dense_min(const Tiling& tl)
+{
+    if (!tl.mfma) return 0;
+    return row16_supported(tl) ? 5 * tl.K / 2 : sparse_tile_max(tl);
+}
+
+// The row gather of the sparse items (P = 16, sub-grid sides 19, 23 and 27: tile sides
 // 4, 8 and 12); false if the sub-grid side has no instance.
-static bool interp_quad_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
+static bool interp_row16_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                               const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
                               double* phi, double* force, const SysDesc* sys, int dense_min, hipStream_t s)
 {
@@ -1195,8 +1209,8 @@ static bool interp_quad_modes(const Rec<float>* rec, const int* item_tile, const
     dim3 gr(tl.max_items), bl(512);
 #define SE_GQ(SV, POT, FORCE)                                                                                 \
     do {                                                                                                      \
-        allow_lds_p((const void*)k_interp_quad<SV, POT, FORCE>, lds);                                         \
-        hipLaunchKernelGGL((k_interp_quad<SV, POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg,         \
+        allow_lds_p((const void*)k_interp_row16<SV, POT, FORCE>, lds);                                         \
+        hipLaunchKernelGGL((k_interp_row16<SV, POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg,         \
                            item_end, n_items, grid, g, tl, N, phi, force, sys, dense_min);                    \
     } while (0)
 #define SE_GQS(SV)                        \
@@ -1230,14 +1244,12 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
     case 11: SE_GPM(11, 0); return true;
     case 16:
         if (tl.mfma) {
-            // A 4-particle MFMA group needs about one particle per pencil row; items with
-            // fewer particles than half the tile's (pencil, row) keys (sparse tiles) go to
-            // the vector gather, the others to the MFMA gather.  The sparse items' gather
-            // is dominated by the sub-grid load, so it also runs two work-groups of at
-            // most 256 threads per CU, one loading while the other computes.  The two
-            // kernels write disjoint particles; given a side stream they run concurrently,
-            // so each fills the other's tail.
-            const int dense_min = mfma_dense_min(tl);
+            // Items below gather_dense_min take the row gather (or, where it has no
+            // instance, the vector gather, which also runs two work-groups of at most 256
+            // threads per CU, one loading its sub-grid while the other computes); the
+            // others take the MFMA gather.  The two kernels write disjoint particles; given
+            // a side stream they run concurrently, so each fills the other's tail.
+            const int dense_min = gather_dense_min(tl);
             Tiling tv = tl;
             tv.block = tl.mblock;
             hipStream_t sv = s;
@@ -1246,7 +1258,7 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
                 SE_HIP(hipStreamWaitEvent(side, fork, 0));
                 sv = side;
             }
-            if (!interp_quad_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
+            if (!interp_row16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
                                    dense_min, sv))
                 interp_pencil_modes<16>(rec, item_tile, item_beg, item_end, n_items, grid, g, tv, N, phi, force, sys,
                                         dense_min, sv);

@@ -259,9 +259,9 @@ def test_mfma_gather_matches_vector_gather(oracle, clustered):
 
 
 @pytest.mark.parametrize("clustered", [False, True])
-def test_quad_gather_of_sparse_items_matches_vector_gather(oracle, clustered):
+def test_row_gather_of_sparse_items_matches_vector_gather(oracle, clustered):
     """Sparse work items at P = 16 (fewer particles than the matrix-core threshold) take
-    the quad gather (four lanes per particle); at every tile side it has an instance for
+    the row gather (a 16-lane row per particle); at every tile side it has an instance for
     (sub-grid sides 27, 23, 19) it agrees with the vector register-ring gather (the
     SE_PLAN_NO_MFMA path) and with the FP64 oracle, for each output mode, and repeats bit
     for bit."""
@@ -290,7 +290,7 @@ def test_quad_gather_of_sparse_items_matches_vector_gather(oracle, clustered):
                 assert all(np.array_equal(u, v) for u, v in zip(out[mf], res)), kw
             out[mf] = res
         a, b = out[True], out[False]
-        print("quad vs vector", kw, [rms_rel(u, v) for u, v in zip(a, b)], flush=True)
+        print("row vs vector", kw, [rms_rel(u, v) for u, v in zip(a, b)], flush=True)
         for u, v in zip(a, b):
             assert rms_rel(u, v) < 2e-6, kw
         assert rms_rel(a[2], a[0]) < 1e-7 and rms_rel(a[3], a[1]) < 1e-7, kw
