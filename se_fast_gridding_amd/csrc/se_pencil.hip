@@ -177,6 +177,35 @@ __device__ __forceinline__ void fmac_row_bcast(float& acc, float y, float x)
                      : "+v"(acc) : "v"(y), "v"(x), "n"(S));
 }
 
+// dst[k] = lane k of v's 16-lane row, for k = 0..15, in every lane (v_mov_b32_dpp
+// row_newbcast); the leading s_nop gives the two wait states a DPP read needs after a
+// VALU write of its source.
+__device__ __forceinline__ void row_bcast16(float v, float (&d)[16])
+{
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_mov_b32_dpp %0, %16 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %1, %16 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %2, %16 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %3, %16 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %4, %16 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %5, %16 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %6, %16 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %7, %16 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %8, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %9, %16 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %10, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %11, %16 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %12, %16 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %13, %16 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %14, %16 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mov_b32_dpp %15, %16 row_newbcast:15 row_mask:0xf bank_mask:0xf"
+        : "=&v"(d[0]), "=&v"(d[1]), "=&v"(d[2]), "=&v"(d[3]), "=&v"(d[4]), "=&v"(d[5]), "=&v"(d[6]), "=&v"(d[7]),
+          "=&v"(d[8]), "=&v"(d[9]), "=&v"(d[10]), "=&v"(d[11]), "=&v"(d[12]), "=&v"(d[13]), "=&v"(d[14]),
+          "=&v"(d[15])
+        : "v"(v));
+}
+
 // Spread FMA block: acc[s][r] += Y_s * xz[r] for all ring slots s, Y_s = lane s of fy's row.
 template <int S, int P, int R>
 struct SpreadRing {
@@ -991,7 +1020,8 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
 // parameter), and sums in registers
 //   c0 = sum_y Fy G,  c1 = sum_y Fy' G                  (2 FMAs per (x, y))
 //   u = sum_x Fx c0,  ux = sum_x Fx' c0,  uy = sum_x Fx c1
-// with F' = -c (t - o h) F the derivative factors (d folded into Fx).  The z contraction
+// with F' = -c (t - o h) F the derivative factors (d folded into Fx; lane j of the row
+// evaluates window index j and the row's lanes share them by DPP broadcasts).  The z contraction
 // comes last: phi = sum_z Fz u, F = sum_z (Fz ux, Fz uy, Fz' u), a four-step DPP sum over
 // the row.  Each step of a half-wave reads two runs of 16 consecutive LDS words (two
 // particles), so bank conflicts stay at most two-way.  A particle costs 140 FMA and 64 LDS
@@ -1044,16 +1074,18 @@ __global__ __launch_bounds__(512, 4) void k_interp_row16(const Rec<float>* __res
         const float4 r = *reinterpret_cast<const float4*>(rec + pi);   // t[0], t[1], t[2], q
         const int lo = rec[pi].lo, orig = rec[pi].orig;
         const int lx = lo & 255, ly = (lo >> 8) & 255, lz = lo >> 16;
+        // lane z of the row evaluates window index z of the x and y factors; every lane
+        // of the row then takes all 16 from the row's lanes
         float fx[P], gx[P], fy[P], gy[P];
-#pragma unroll
-        for (int j = 0; j < P; ++j) {
-            const float o = (float)(g.first + j) * h;
+        {
+            const float o = (float)(g.first + z) * h;
             const float ax = r.x - o, ay = r.y - o;
-            fx[j] = gauss(kc, ax) * dn;
-            fy[j] = gauss(kc, ay);
+            const float fxz = gauss(kc, ax) * dn, fyz = gauss(kc, ay);
+            row_bcast16(fxz, fx);
+            row_bcast16(fyz, fy);
             if (FORCE) {
-                gx[j] = fx[j] * (-c * ax);
-                gy[j] = fy[j] * (-c * ay);
+                row_bcast16(fxz * (-c * ax), gx);
+                row_bcast16(fyz * (-c * ay), gy);
             }
         }
         const float* base = sub + (ly * SS + lx) * SS + lz + z;
