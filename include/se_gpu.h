@@ -77,6 +77,10 @@ typedef struct {
  * keeps each (x, z) column of the tile's sub-grid in registers (for A/B comparisons and
  * tests). */
 #define SE_PLAN_NO_PLANE_SPREAD 4
+/* Run the dense work items of the P = 16 FP32 path on the 4 x 1 pencil kernels instead of
+ * the 1 x 1 column kernels, whose matrix-core products take sixteen particles of one
+ * window-start column at a time (for A/B comparisons and tests). */
+#define SE_PLAN_NO_COLUMN 8
 
 int se_plan_create(se_plan** plan, const se_plan_desc* desc, void* hip_stream);
 int se_plan_destroy(se_plan* plan);

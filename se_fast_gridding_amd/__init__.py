@@ -67,6 +67,7 @@ class se_plan_desc(C.Structure):
 PLAN_GENERIC_KERNELS = 1
 PLAN_NO_MFMA = 2
 PLAN_NO_PLANE_SPREAD = 4
+PLAN_NO_COLUMN = 8
 
 
 class se_job_shape(C.Structure):
@@ -305,12 +306,13 @@ class Plan:
     solve_group."""
 
     def __init__(self, M, P, box, xi, n_max, precision=PREC_FP64, m=0.0, tile=0, chunk=0, block=0,
-                 stream=None, generic=False, periodicity=3, slab=0.0, Mz=0, max_systems=1, mfma=True, plane=True):
+                 stream=None, generic=False, periodicity=3, slab=0.0, Mz=0, max_systems=1, mfma=True, plane=True,
+                 column=True):
         import torch
         self._torch = torch
         d = se_plan_desc(periodicity, M, P, float(box), float(xi), float(m), precision, int(n_max), tile, chunk,
                          block, (PLAN_GENERIC_KERNELS if generic else 0) | (0 if mfma else PLAN_NO_MFMA)
-                         | (0 if plane else PLAN_NO_PLANE_SPREAD),
+                         | (0 if plane else PLAN_NO_PLANE_SPREAD) | (0 if column else PLAN_NO_COLUMN),
                          float(slab), int(Mz))
         self.desc = d
         self.max_systems = int(max_systems)

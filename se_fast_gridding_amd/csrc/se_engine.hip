@@ -56,7 +56,8 @@ struct DevMem {
 // sub-grids hold the grid precision.  Default tile side: the largest (<= 16; a multiple
 // of 4 and <= 12 for the pencil kernels) whose FP64 sub-grid fits in LDS.
 static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sys, int tile, int chunk, int block,
-                            int lds_max, bool allow_pencil, bool allow_mfma, bool allow_plane)
+                            int lds_max, bool allow_pencil, bool allow_mfma, bool allow_plane,
+                            bool allow_column)
 {
     Tiling tl{};
     tl.pencil = (allow_pencil && esize == 4 && pencil_supported(P)) ? 1 : 0;
@@ -114,6 +115,8 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     // plane spread of sparse tiles: one lane per (x, z) column of the sub-grid, 32 z columns
     // per wave (S <= 32), its y values in registers (tile side <= 12)
     tl.plane = (tl.pencil && tl.TS <= 12 && tl.S <= 32 && allow_plane) ? 1 : 0;
+    // column kernels for the dense items (matrix-core gather only)
+    tl.column = (tl.mfma && allow_column) ? 1 : 0;
     // every launch shape must lie within its kernel's __launch_bounds__: refused here, at
     // plan creation, rather than by the launch guards at solve time
     SE_REQUIRE(tl.sblock <= spread_pencil_max_threads(P) && tl.mblock <= 512 && tl.block <= 512, SE_ERR_ARG,
@@ -439,7 +442,7 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
     if (tile == 0 && chunk == 0 && block == 0) tuned_tiling(d, std::max(d->n_max, 1) / n_sys, tile, chunk, block);
     Tiling tl = choose_tiling(es, d->P, g.M, g.Mz, d->n_max, n_sys, tile, chunk, block, lds_max,
                               !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA),
-                              !(d->flags & SE_PLAN_NO_PLANE_SPREAD));
+                              !(d->flags & SE_PLAN_NO_PLANE_SPREAD), !(d->flags & SE_PLAN_NO_COLUMN));
     const size_t M = g.M, Mz = g.Mz, N = d->n_max, ns = n_sys;
     size_t b = 0;
     b += 4 * N * 4;                                       // two (key, index) pairs of the key sort
@@ -718,7 +721,7 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         if (tile == 0 && chunk == 0 && block == 0) tuned_tiling(d, pl->n_cap / n_sys, tile, chunk, block);
         pl->tl = choose_tiling(pl->esize, d->P, pl->g.M, pl->g.Mz, pl->n_cap, n_sys, tile, chunk, block, lds_max,
                                !(d->flags & SE_PLAN_GENERIC_KERNELS), !(d->flags & SE_PLAN_NO_MFMA),
-                               !(d->flags & SE_PLAN_NO_PLANE_SPREAD));
+                               !(d->flags & SE_PLAN_NO_PLANE_SPREAD), !(d->flags & SE_PLAN_NO_COLUMN));
         const Tiling& tl = pl->tl;
         const size_t M = pl->g.M, Mz = pl->g.Mz, N = pl->n_cap, ns = n_sys;
         SE_REQUIRE(ns * tl.nbins < (1ul << 31) - 1 && ns * tl.ntiles * (size_t)tl.K < (1ul << 31) - 1, SE_ERR_ARG,

@@ -82,6 +82,7 @@ struct Tiling {
     int sblock;       // threads per work-group of the register-ring (pencil) spread
     int block_auto;   // 1: block was not given, so each kernel may take its own size
     int plane;        // 1: sparse tiles take the plane spread (pencil spreads only)
+    int column;       // 1: dense items take the 1 x 1 column kernels (P = 16 FP32, with mfma)
 };
 
 // Walk over the S^3 points of a tile sub-grid, e = (a S + b) S + c, visited as e = tid,
@@ -344,6 +345,11 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
                           const Geom& g, const Tiling& tl, int N, double* phi, double* force,
                           const SysDesc* sys, hipStream_t s, hipStream_t side = nullptr,
                           hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
+// column gather (se_column.hip): items of at least dense_min particles, P = 16 FP32,
+// work-groups of 256 or 512 threads
+void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
+                         const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
+                         double* phi, double* force, const SysDesc* sys, int dense_min, int block, hipStream_t s);
 template <typename T>
 void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const Geom& g,

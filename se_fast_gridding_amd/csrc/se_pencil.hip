@@ -1295,8 +1295,14 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
                 interp_pencil_modes<16>(rec, item_tile, item_beg, item_end, n_items, grid, g, tv, N, phi, force, sys,
                                         dense_min, sv);
             if (side) SE_HIP(hipEventRecord(join, side));
-            interp_mfma16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
-                                dense_min, s);
+            // the column gather at 512 threads: 4 waves per SIMD (two work-groups per CU by
+            // LDS); clustered N = 4M at M = 256: 256 threads 1.17 ms, 512 0.99, 1024 1.08
+            if (tl.column)
+                launch_interp_col16(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
+                                    dense_min, 512, s);
+            else
+                interp_mfma16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
+                                    dense_min, s);
             if (side) SE_HIP(hipStreamWaitEvent(s, join, 0));
         } else {
             SE_GPM(16, 0);

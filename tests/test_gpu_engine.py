@@ -259,6 +259,48 @@ def test_mfma_gather_matches_vector_gather(oracle, clustered):
 
 
 @pytest.mark.parametrize("clustered", [False, True])
+def test_column_kernels_match_pencil_kernels(oracle, clustered):
+    """The P = 16 column kernels (sixteen particles of one 1 x 1 window-start column per
+    matrix-core product) against the 4 x 1 pencil kernels (SE_PLAN_NO_COLUMN) and the FP64
+    oracle, for each output mode, with items that cut columns mid-way (chunk 45 and 301,
+    tile sides 12, 8 and 4, dense thresholds 1080, 360 and 40 particles), work-groups of
+    64 to 512 threads; and bit for bit on a repeat."""
+    N, M, P, xi = 6000, 32, 16, 8.0
+    x, q = system(N, 320 + int(clustered), clustered=clustered)
+    m = 0.95 * np.sqrt(np.pi * P)
+    ref_phi = oracle.se3p(x, q, 1.0, M, P, m, xi)
+    ref_F = oracle.se3p(x, q, 1.0, M, P, m, xi, force=True)
+    xt, qt = torch.tensor(x, device=DEV), torch.tensor(q, device=DEV)
+    for kw in (dict(), dict(chunk=45, block=128), dict(tile=8, chunk=301), dict(tile=4, chunk=301, block=64)):
+        out = {}
+        for col in (True, False, True):
+            print("case column=%s %s" % (col, kw), flush=True)
+            plan = se.Plan(M, P, 1.0, xi, N, precision=se.PREC_FP32, column=col, **kw)
+            phi = torch.zeros(N, dtype=torch.float64, device=DEV)
+            F = torch.zeros(3 * N, dtype=torch.float64, device=DEV)
+            p1 = torch.zeros(N, dtype=torch.float64, device=DEV)
+            f1 = torch.zeros(3 * N, dtype=torch.float64, device=DEV)
+            plan.solve(xt, qt, phi, F)
+            plan.interp(p1, None)
+            plan.interp(None, f1)
+            torch.cuda.synchronize()
+            assert plan.info()["pencil"] == 1
+            plan.close()
+            res = [t.cpu().numpy() for t in (phi, F, p1, f1)]
+            if col in out:
+                assert all(np.array_equal(u, v) for u, v in zip(out[col], res)), kw
+            out[col] = res
+        a, b = out[True], out[False]
+        print("column vs pencil", kw, [rms_rel(u, v) for u, v in zip(a, b)], flush=True)
+        for u, v in zip(a, b):
+            assert rms_rel(u, v) < 2e-6, kw
+        print("potential-only, force-only vs joint: equal %s %s" % (np.array_equal(a[2], a[0]),
+                                                                    np.array_equal(a[3], a[1])), flush=True)
+        assert rms_rel(a[2], a[0]) < 1e-7 and rms_rel(a[3], a[1]) < 1e-7, kw
+        assert rms_rel(a[0], ref_phi) < FP32_TOL and rms_rel(a[1], ref_F) < FP32_TOL_F, kw
+
+
+@pytest.mark.parametrize("clustered", [False, True])
 def test_row_gather_of_sparse_items_matches_vector_gather(oracle, clustered):
     """Sparse work items at P = 16 (fewer particles than the matrix-core threshold) take
     the row gather (a 16-lane row per particle); at every tile side it has an instance for
