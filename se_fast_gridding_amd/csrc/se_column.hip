@@ -249,6 +249,291 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
     if (have > 0) process(have);
 }
 
+// ---------------------------------------------------------------------------------
+// Column spread of the dense items (P = 16).  The same 1 x 1 columns: for a grid row y,
+// four particles k of one column add
+//
+//   D_y[x][z] += sum_k (q_k d Fx_k(x) Fy_k(y)) Fz_k(z),    x, z = 0..15
+//
+// which is one v_mfma_f32_16x16x4f32 (A = the particles' x factors times their y factor
+// at row y, B = their z factors; K = the four particles).  A wave keeps a ring of 20 grid
+// rows of its current column in registers (D of row y in ring slot y mod 20, four VGPRs
+// each).  A group is up to sixteen particles of one column whose windows lie in the ring
+// [y0, y0 + 20), y0 the first one's row: four K steps of four, each 20 matrix
+// instructions with no branch.  Lane i of particle k's 16-lane row holds the y factor of
+// ring slot i (and of slot 16 + i), so the A operand of slot s is one DPP multiply
+// (row_newbcast: s mod 16) with every register index static.  Rows that leave the ring are
+// added to the tile's FP64 LDS sub-grid: each value, scaled by 2^s per work item (q d 2^s
+// is folded into the x factor) and rounded to an integer, is added exactly, so the sub-grid
+// sums do not depend on the order of the atomics; the ring's own FP32 sums run in the fixed
+// order of the groups.  The sub-grid then becomes the item's FP32 partial, as in the pencil
+// spread.  Holding all S rows of the column instead, with a uniform branch per row and K
+// step, measured slower (clustered N = 4M at M = 256, dense items: 1.03 against 0.93 ms):
+// the branches cost more than the ring's extra rows.
+//
+// Work split: as in the column gather, four waves to a share of the item's records, each
+// taking one column of every pencil of the share (rotated from pencil to pencil).  A
+// group's particles are taken from the wave's current batch of 64 records (one record per
+// lane; the next batch is loaded meanwhile): their batch lanes go to a per-wave slot list in
+// LDS, and the lanes of each particle fetch its fields with ds_bpermute and form its factors.
+// ---------------------------------------------------------------------------------
+constexpr int COL_RING = 20;
+constexpr int COL_KSTEPS = 4;   // K steps (of four particles) per group
+
+// a * (lane L of b's 16-lane row): a DPP row_newbcast:L move (GFX9.4 DPP) into a multiply,
+// which the compiler folds into one v_mul_f32_dpp
+template <int L>
+__device__ __forceinline__ float mul_row_bcast(float a, float b)
+{
+    return a * __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(b), 0x150 + L, 0xF, 0xF, false));
+}
+
+// one K step: ring slot s += (qfx * Y_s) (x) fz, Y_s = lane s mod 16 of fy1 (s < 16) or fy2
+template <int S0>
+struct ColumnRing {
+    static __device__ __forceinline__ void run(v4f (&acc)[COL_RING], float qfx, float fy1, float fy2, float fz)
+    {
+        const float a = S0 < 16 ? mul_row_bcast<S0 & 15>(qfx, fy1) : mul_row_bcast<S0 & 15>(qfx, fy2);
+        acc[S0] = mfma16(a, fz, acc[S0]);
+        ColumnRing<S0 + 1>::run(acc, qfx, fy1, fy2, fz);
+    }
+};
+template <>
+struct ColumnRing<COL_RING> {
+    static __device__ __forceinline__ void run(v4f (&)[COL_RING], float, float, float, float) {}
+};
+
+// Ring slots holding rows [lo, lo + n) (0 <= n <= R) as a bit mask: slot s holds the row
+// == s (mod R), so the mask is n consecutive bits rotated by lo mod R.
+template <int R>
+__device__ __forceinline__ unsigned ring_slots(int lo, int n)
+{
+    const unsigned m = n >= R ? (1u << R) - 1u : (1u << n) - 1u;
+    const int b = lo % R;
+    return ((m << b) | (m >> (R - b))) & ((1u << R) - 1u);
+}
+
+__global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restrict__ rec,
+                                                      const int* __restrict__ item_beg,
+                                                      const int* __restrict__ item_end,
+                                                      const int* __restrict__ n_items, Geom g, Tiling tl,
+                                                      float* __restrict__ partials, int dense_min)
+{
+    constexpr int P = 16, GY = COL_RING, KS = COL_KSTEPS, GMAX = 4 * KS;
+    const int item = blockIdx.x;
+    if (item >= *n_items) return;
+    const int beg = item_beg[item], end = item_end[item], n = end - beg;
+    // items below dense_min particles are the pencil and plane spreads'
+    if (n < dense_min) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ float wmax[12];
+    __shared__ int slots[12 * 16];   // per wave: the batch lanes of the open group's particles
+    double* sub = reinterpret_cast<double*>(smem_raw);   // [y][x][z], S^3
+    const int S = tl.S, S2 = S * S, S3 = S2 * S;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int e = tid; e < S3; e += nthr) sub[e] = 0.0;
+    item_qmax_part(rec, beg, end, tid, nthr, wmax);
+    __syncthreads();
+
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
+    const int col = wave & 3, part = wave >> 2, nparts = nw >> 2;
+    const int rb = beg + (int)(((long)n * part) / nparts), re = beg + (int)(((long)n * (part + 1)) / nparts);
+    int* myslots = slots + 16 * wave;
+    const int i16 = lane & 15, k4 = lane >> 4;   // lane = i16 + 16 k4: x / z index, particle slot
+    // fixed-point scale of the item's LDS sums: q d 2^sc is folded into the x factor
+    const int sc = __builtin_amdgcn_readfirstlane(flush_scale((double)n * item_qmax_read(wmax, nw) * g.d, g.d));
+    const float h = (float)g.h, c = (float)g.c;
+    const float dn = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)ldexp(g.d, sc))));
+    const float kc = -c * 1.4426950408889634f;
+    const float oi = (float)(g.first + i16) * h;   // grid offset of window index i16
+
+    v4f acc[GY];
+#pragma unroll
+    for (int s = 0; s < GY; ++s) acc[s] = v4f{0.f, 0.f, 0.f, 0.f};
+    int rpen = -1, base = 0, rofs = 0;   // ring: column key, first row, this lane's point (x0 + 4 k4, 0, z0 + i16)
+
+    // add ring rows [base, upto) (upto <= base + GY; rows past the sub-grid hold zeros)
+    // into LDS and clear their slots
+    auto flush = [&](int upto) __attribute__((always_inline)) {
+        upto = min(upto, S);
+        if (upto <= base) return;
+        const unsigned mask = ring_slots<GY>(base, upto - base);
+        const int b = base % GY;
+#pragma unroll
+        for (int s = 0; s < GY; ++s) {
+            if (mask & (1u << s)) {
+                // a distinct block per slot (so that the slots' blocks are not merged into
+                // one that indexes the ring registers at run time)
+                asm volatile("; ring slot %0" ::"n"(s));
+                const int y = base + (s >= b ? s - b : s - b + GY);
+                double* p = sub + y * S2 + rofs;
+                const v4f v = acc[s];
+                acc[s] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    __hip_atomic_fetch_add(p + r * S, (double)rintf(v[r]), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    };
+
+    // The open group: particle 4 cc + k4 of K step cc has its factors on this lane's row,
+    // formed when it joins the group: x factor (times q d 2^sc; zero on slots without a
+    // particle), z factor, y factors of ring slots i16 and 16 + (i16 & 3) for the ring
+    // starting at the group's first row.
+    float gqx[KS], gfz[KS], gy1[KS], gy2[KS];
+#pragma unroll
+    for (int cc = 0; cc < KS; ++cc) gqx[cc] = gfz[cc] = gy1[cc] = gy2[cc] = 0.f;
+
+    auto process = [&](int k, int gpen, int gy0, int lo0) __attribute__((always_inline)) {
+        // one flush site (rows [base, upto) of the old ring), so the ring registers are
+        // never copied between control-flow paths
+        int upto = base;
+        if (gpen != rpen) {
+            if (rpen >= 0) upto = base + GY;
+        } else if (gy0 > base) {
+            upto = min(gy0, base + GY);
+        }
+        flush(upto);
+        if (gpen != rpen) {
+            rpen = gpen;
+            rofs = ((lo0 & 255) + 4 * k4) * S + (lo0 >> 16) + i16;
+        }
+        base = gy0;
+#pragma unroll
+        for (int cc = 0; cc < KS; ++cc) {
+            if (4 * cc < k) ColumnRing<0>::run(acc, gqx[cc], gy1[cc], gy2[cc], gfz[cc]);
+            gqx[cc] = 0.f;   // the next group starts with empty slots
+        }
+    };
+
+    // Scan (as in the column gather); the next batch of records is loaded while the current
+    // one is processed.  Each pass of the outer loop fills one group and processes it at a
+    // single call site.
+    int have = 0, gpen = -1, gy0 = 0, glo0 = 0, pcarry = 0, ppen = -1;
+    if (lane < 16) myslots[lane] = 0;   // every slot names a lane before its first fill
+    int cb = rb - 64;                    // current batch (none yet)
+    float4 ra = {0.f, 0.f, 0.f, 0.f}, ra_next = {0.f, 0.f, 0.f, 0.f};
+    int lo = -1, lo_next = -1, pen = 0, ly = 0;
+    bool match = false;
+    unsigned long long m = 0;
+    if (rb + lane < re) {
+        ra_next = *reinterpret_cast<const float4*>(rec + rb + lane);
+        lo_next = rec[rb + lane].lo;
+    }
+    for (;;) {
+        bool close = false;
+        while (!close) {
+            if (m == 0) {
+                if (cb + 64 >= re) break;   // no records left
+                cb += 64;
+                const int i = cb + lane;
+                ra = ra_next;
+                lo = lo_next;
+                lo_next = -1;
+                if (cb + 64 + lane < re) {
+                    ra_next = *reinterpret_cast<const float4*>(rec + cb + 64 + lane);
+                    lo_next = rec[cb + 64 + lane].lo;
+                }
+                const bool in = i < re;
+                pen = lo & 0xFF00FC;
+                ly = (lo >> 8) & 255;
+                const int prev = __shfl_up(pen, 1, 64);
+                const bool start = in && pen != (lane == 0 ? ppen : prev);
+                const unsigned long long smask = __ballot(start);
+                const int pord = pcarry - 1 +
+                                 (int)__builtin_amdgcn_mbcnt_hi((unsigned)(smask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((unsigned)smask, 0u)) +
+                                 (start ? 1 : 0);
+                pcarry += __builtin_popcountll(smask);
+                ppen = __builtin_amdgcn_readlane(pen, 63);
+                match = in && (lo & 3) == ((col + pord) & 3);
+                m = __ballot(match);
+                continue;
+            }
+            const int j = __builtin_ctzll(m);
+            const int pj = __builtin_amdgcn_readlane(pen, j), lyj = __builtin_amdgcn_readlane(ly, j);
+            if (have > 0 && (pj != gpen || lyj > gy0 + (GY - P))) break;   // the group is complete
+            if (have == 0) {
+                gpen = pj;
+                gy0 = lyj;
+                glo0 = __builtin_amdgcn_readlane(lo, j);
+            }
+            // this pencil's next matches whose windows fit the ring from gy0 (a run: the
+            // records are sorted by pencil, then row)
+            const unsigned long long same = __ballot(match && pen == gpen && ly <= gy0 + (GY - P)) & m;
+            const int take = min(__builtin_popcountll(same), GMAX - have);
+            const unsigned long long taken = first_bits(same, take);
+            if ((taken >> lane) & 1ull) {
+                const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(taken >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)taken, 0u));
+                myslots[have + rank] = lane;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // the lanes of particle 4 cc + k4 fetch its fields from the batch lane in its slot
+            // and form its factors; the ring will start at row gy0 (slot s holds the row
+            // == s (mod GY) of [gy0, gy0 + GY))
+            const int bm = gy0 % GY;
+            const int t1 = i16 - bm, t2 = 16 + (i16 & 3) - bm;
+            const int y1 = gy0 + (t1 < 0 ? t1 + GY : t1), y2 = gy0 + (t2 < 0 ? t2 + GY : t2);
+#pragma unroll
+            for (int cc = 0; cc < KS; ++cc) {
+                if (4 * cc + 3 >= have && 4 * cc < have + take) {   // K step cc gains particles
+                    const int p = 4 * cc + k4;
+                    const int sl = 4 * myslots[p];
+                    const bool fill = p >= have && p < have + take;
+                    const float t0 = __int_as_float(__builtin_amdgcn_ds_bpermute(sl, __float_as_int(ra.x)));
+                    const float ty = __int_as_float(__builtin_amdgcn_ds_bpermute(sl, __float_as_int(ra.y)));
+                    const float t2f = __int_as_float(__builtin_amdgcn_ds_bpermute(sl, __float_as_int(ra.z)));
+                    const float q = __int_as_float(__builtin_amdgcn_ds_bpermute(sl, __float_as_int(ra.w)));
+                    const int lyp = __builtin_amdgcn_ds_bpermute(sl, ly);
+                    const int w1 = y1 - lyp, w2 = y2 - lyp;   // window indices of the two slots' rows
+                    const float qx = q * dn * gauss_c(kc, t0 - oi), fz = gauss_c(kc, t2f - oi);
+                    const float f1 = (unsigned)w1 < (unsigned)P ? gauss_c(kc, ty - (float)(g.first + w1) * h) : 0.f;
+                    const float f2 = (unsigned)w2 < (unsigned)P ? gauss_c(kc, ty - (float)(g.first + w2) * h) : 0.f;
+                    if (fill) {
+                        gqx[cc] = qx;
+                        gfz[cc] = fz;
+                        gy1[cc] = f1;
+                        gy2[cc] = f2;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            have += take;
+            m &= ~taken;
+            close = have == GMAX;
+        }
+        if (have == 0) break;
+        process(have, gpen, gy0, glo0);
+        have = 0;
+    }
+    if (rpen >= 0) flush(base + GY);
+    __syncthreads();
+    float* out = partials + (size_t)item * tl.part_elems;
+    SubWalk w(S, tid, nthr);   // canonical [x][y][z]: a = ix, b = iy, c = iz
+    for (int e = tid; e < S3; e += nthr) {
+        out[e] = (float)ldexp(sub[(w.b * S + w.a) * S + w.c], -sc);
+        w.next();
+    }
+}
+
+// Column spread of the items of at least dense_min particles: work-groups of `block`
+// threads (256, 512 or 768; the kernel is held to 168 registers, 3 waves per SIMD), one
+// per CU (the FP64 sub-grid fills the LDS).
+bool launch_spread_col16(const Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items,
+                         const Geom& g, const Tiling& tl, float* partials, int dense_min, int block, hipStream_t s)
+{
+    SE_REQUIRE(g.P == 16 && tl.S >= 16 + tl.TS - 1 && block % 256 == 0 && block >= 256 && block <= 768, SE_ERR_ARG,
+               "column spread: P = 16 and 256, 512 or 768 threads");
+    const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(double);
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_spread_col16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_spread_col16, dim3(tl.max_items), dim3(block), lds, s, rec, item_beg, item_end, n_items, g,
+                       tl, partials, dense_min);
+    return true;
+}
+
 // Column gather of the items of at least dense_min particles; block = 256, 512 or 1024.
 void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                          const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,

@@ -350,6 +350,10 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
 void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                          const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
                          double* phi, double* force, const SysDesc* sys, int dense_min, int block, hipStream_t s);
+// column spread (se_column.hip): items of at least dense_min particles, P = 16 FP32,
+// work-groups of 256, 512 or 768 threads; false if the sub-grid side has no instance
+bool launch_spread_col16(const Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items,
+                         const Geom& g, const Tiling& tl, float* partials, int dense_min, int block, hipStream_t s);
 template <typename T>
 void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const Geom& g,

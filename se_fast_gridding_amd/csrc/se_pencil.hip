@@ -264,14 +264,17 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
                                                        const int* __restrict__ item_end,
                                                        const int* __restrict__ n_items, Geom g, Tiling tl,
                                                        float* __restrict__ partials, const int* __restrict__ item_tile,
-                                                       const int* __restrict__ tile_start, int sparse_max)
+                                                       const int* __restrict__ tile_start, int sparse_max,
+                                                       int dense_min)
 {
     constexpr int R = ((P + 3) * P + 63) / 64;
     const int item = blockIdx.x;
     if (item >= *n_items) return;   // uniform across the work-group
     // sparse items (whole tiles of fewer than sparse_max particles) are the plane
-    // spread's (sparse_max > 0)
+    // spread's (sparse_max > 0), items of dense_min particles or more the column spread's
+    // (dense_min > 0)
     if (sparse_item(item_tile, item_beg, item_end, tile_start, item, sparse_max)) return;
+    if (dense_min > 0 && item_end[item] - item_beg[item] >= dense_min) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ float wmax[spread_pencil_max_threads(P) / 64];
     double* sub = reinterpret_cast<double*>(smem_raw);   // [y][x][z], S^3
@@ -1144,6 +1147,10 @@ static void allow_lds_p(const void* fn, size_t bytes)
 // the pencil spread; 0 turns the plane spread off.
 int spread_sparse_max(const Tiling& tl) { return tl.plane ? sparse_tile_max(tl) : 0; }
 
+// Items of this many particles or more take the column spread (0: none); the gather's
+// threshold for its column kernel.
+static int spread_dense_min(const Tiling& tl, int P) { return (tl.column && P == 16) ? 5 * tl.K / 2 : 0; }
+
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const int* tile_start, const Geom& g, const Tiling& tl, float* partials,
                           hipStream_t s)
@@ -1152,12 +1159,15 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int
     SE_REQUIRE(tl.sblock >= 64 && tl.sblock % 64 == 0 && tl.sblock <= spread_pencil_max_threads(g.P), SE_ERR_ARG,
                "spread work-group size past the kernel's bound");
     const int sparse_max = spread_sparse_max(tl);
+    int dense_min = spread_dense_min(tl, g.P);
+    if (dense_min > 0 && !launch_spread_col16(rec, item_beg, item_end, n_items, g, tl, partials, dense_min, 768, s))
+        dense_min = 0;   // no column spread for this sub-grid side: the pencil spread takes every item
     dim3 gr(tl.max_items), bl(tl.sblock);
 #define SE_SP(PV)                                                                                            \
     do {                                                                                                     \
         allow_lds_p((const void*)k_spread_pencil<PV>, lds);                                                  \
         hipLaunchKernelGGL(k_spread_pencil<PV>, gr, bl, lds, s, rec, item_beg, item_end, n_items, g, tl,    \
-                           partials, item_tile, tile_start, sparse_max);                                      \
+                           partials, item_tile, tile_start, sparse_max, dense_min);                           \
         if (sparse_max > 0)                                                                                  \
             hipLaunchKernelGGL(k_spread_plane<PV>, gr, dim3(64 * ((tl.S + 1) / 2)), 0, s, rec, item_tile,      \
                                item_beg, item_end, n_items, tile_start, g, tl, partials, sparse_max);         \
