@@ -1,30 +1,28 @@
 // Column kernels: the FP32 P = 16 path for dense work items on the matrix cores.
 //
 // Particles are sorted (tile, pencil, row), a pencil being 4 x 1 window-start columns
-// (4 in x, 1 in z) walked along y.  The particles of one 1 x 1 column of a pencil (one
-// lx, one lz) share their x and z windows exactly, so for any grid row y the z
-// contraction of up to sixteen of them is one 16 x 16 x 16 matrix product:
+// (4 in x, 1 in z) walked along y.  The particles of one 1 x 1 column (one lx, one lz)
+// share their x and z windows exactly, so for any grid row y the z contraction of up to
+// sixteen of them is one 16 x 16 x 16 matrix product:
 //
 //   T_y[x][n] = sum_z G[y][x0 + x][z0 + z] Fz_n(z),     x, z = 0..15, n = particle
 //
 // which v_mfma_f32_16x16x4f32 runs as four K = 4 steps (A = the sub-grid slice, read
-// from LDS; B = the particles' z factors).  Every product is a useful one: no window
-// padding, no masking.  The y and x contractions are per particle and stay on the VALU:
+// from LDS; B = the particles' z factors), and the spread's rank-1 updates of one row,
+// four particles at a time, are one such instruction too.  Every product is a useful one:
+// no window padding, no masking.
 //
-//   U = sum_y Fy T,  V = sum_y Fy' T,  W = sum_y Fy T'   (T' with the z derivative factors)
-//   phi = sum_x Fx U,  F = (sum_x Fx' U, sum_x Fx V, sum_x Fx W)
+// A dense item's records are first put in column order, (pencil, lx mod 4, row), by
+// k_column_order (a stable partition of each pencil by column), so that a column's
+// particles are consecutive records in row order: a group is a run of up to sixteen
+// consecutive records of one column, and the work of an item splits into equal record
+// shares per wave.  Dense items are those of dense_min to COLUMN_MAX particles
+// (column_item); the other kernels of the spread and the gather skip exactly these.
 //
-// A group costs (rows spanned + 15) row steps of 4 (potential) or 8 (potential and
-// force) matrix instructions, whatever its 1..16 particles; in a dense column, sixteen
-// consecutive particles span one or two rows, so nearly every instruction is full.
-//
-// Work split: the waves of a work-group (4, 8 or 16) take the item's records in 1, 2 or 4
-// equal shares, four waves to a share; each of the four takes one column of every pencil
-// of the share, scanning the records in sort order with ballots; a group closes at 16
-// particles or at a pencil boundary.  The
-// particles of a group come in row order, so rows y0 .. y1 + 15 cover every window.
-// Each particle's result depends only on its own record and on the sub-grid: rows outside
-// its window weigh exactly zero, so the grouping does not change any output bit.
+// FP32 matrix instructions and FP32 vector instructions do not overlap on this GPU (one
+// v_mfma_f32_16x16x4f32 and eight v_fma_f32 per iteration take the sum of their times, at
+// two or four waves per SIMD), so the kernels keep the vector work per matrix
+// instruction small rather than counting on it to hide behind them.
 #include <hip/hip_runtime.h>
 
 #include "se_internal.h"
@@ -54,21 +52,163 @@ __device__ __forceinline__ v4f mfma16(float a, float b, v4f c)
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Lane masks of the first k set bits of m (k <= popcount(m)).
-__device__ __forceinline__ unsigned long long first_bits(unsigned long long m, int k)
+__device__ __forceinline__ float bperm_f(int byte_addr, float v)
 {
-    unsigned long long rest = m;
-#pragma nounroll
-    for (int t = 0; t < k; ++t) rest &= rest - 1;
-    return m & ~rest;
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(byte_addr, __float_as_int(v)));
+}
+
+// Exclusive scan over the work-group of one value per thread (wave scans by shuffles,
+// then the wave totals through LDS); returns the combination of the threads before this
+// one (ident for thread 0).
+template <typename T, typename Op>
+__device__ __forceinline__ T block_exclusive(T v, T ident, Op op, T* wtot)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc = op(inc, u);
+    }
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    T before = ident;
+    for (int w = 0; w < wave; ++w) before = op(before, wtot[w]);
+    __syncthreads();
+    const T ex = __shfl_up(inc, 1, 64);
+    return op(before, lane == 0 ? ident : ex);
 }
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------
+// Column order of the dense items: each pencil's records, in row order, are partitioned
+// stably by their column (lx mod 4), so the records become (pencil, column, row).  One
+// work-group per item; each thread holds eight consecutive records in registers, the
+// records' window words sit in LDS for the neighbour tests, and record i goes to
+//   s + (records of the pencil in lower columns) + (records of its column before it),
+// s the pencil's first record, from per-column running counts (four 16-bit fields of one
+// 64-bit count, scanned over the item) taken at the pencil's start and end.  Every record
+// is read before the first barrier and written after the last, so the permutation is
+// done in place.
+// ---------------------------------------------------------------------------------
+constexpr int COLUMN_ORDER_THREADS = 256;
+constexpr int COLUMN_PER_THREAD = COLUMN_MAX / COLUMN_ORDER_THREADS;
+
+__device__ __forceinline__ unsigned long long col_one(int lo) { return 1ull << (16 * (lo & 3)); }
+__device__ __forceinline__ int col_field(unsigned long long c, int d) { return (int)((c >> (16 * d)) & 0xFFFF); }
+
+__global__ __launch_bounds__(COLUMN_ORDER_THREADS) void k_column_order(Rec<float>* __restrict__ rec,
+                                                                       const int* __restrict__ item_beg,
+                                                                       const int* __restrict__ item_end,
+                                                                       const int* __restrict__ n_items,
+                                                                       int dense_min)
+{
+    const int item = blockIdx.x;
+    if (item >= *n_items) return;
+    const int beg = item_beg[item], n = item_end[item] - beg;
+    if (!column_item(n, dense_min)) return;
+    __shared__ int los[COLUMN_MAX];                        // the records' window words
+    __shared__ unsigned long long cst[COLUMN_MAX];         // per-column counts before a pencil's first record
+    __shared__ unsigned long long cen[COLUMN_MAX];         // ... and after its last (indexed by its first)
+    __shared__ unsigned int wlo[16], whi[16];
+    __shared__ int wmx[16];
+    const int t = threadIdx.x;
+    const int i0 = t * COLUMN_PER_THREAD;   // thread t owns records i0 .. i0 + COLUMN_PER_THREAD - 1
+    v4f* rp = reinterpret_cast<v4f*>(rec + beg);   // plain vectors (no HIP vector-type copies)
+    v4f ra[COLUMN_PER_THREAD], rb[COLUMN_PER_THREAD];
+#pragma unroll
+    for (int u = 0; u < COLUMN_PER_THREAD; ++u) {
+        const int i = min(i0 + u, n - 1);   // threads past the item reread its last record
+        ra[u] = rp[2 * i];
+        rb[u] = rp[2 * i + 1];   // lo, orig, pad
+        if (i0 + u < n) los[i] = __float_as_int(rb[u].x);
+    }
+    __syncthreads();
+    // record i starts its pencil (lx / 4, lz)
+#define SE_STARTS(i, lo) ((i) == 0 || ((los[(i) - 1] ^ (lo)) & 0xFF00FC) != 0)
+
+    unsigned long long cl[COLUMN_PER_THREAD];
+    unsigned long long cnt = 0;
+    int last = -1;
+#pragma unroll
+    for (int u = 0; u < COLUMN_PER_THREAD; ++u) {
+        const int i = i0 + u;
+        cl[u] = cnt;
+        if (i < n) {
+            const int lo = __float_as_int(rb[u].x);
+            cnt += col_one(lo);
+            if (SE_STARTS(i, lo)) last = i;
+        }
+    }
+    // counts before this thread's records (a field counts at most COLUMN_MAX < 2^16, so
+    // no field carries into the next, and the two 32-bit halves are scanned separately)
+    const unsigned int blo = block_exclusive<unsigned int>((unsigned int)cnt, 0u,
+                                                           [](unsigned int a, unsigned int b) { return a + b; }, wlo);
+    const unsigned int bhi = block_exclusive<unsigned int>((unsigned int)(cnt >> 32), 0u,
+                                                           [](unsigned int a, unsigned int b) { return a + b; }, whi);
+    const int prev_start = block_exclusive<int>(last, -1, [](int a, int b) { return a > b ? a : b; }, wmx);
+    const unsigned long long cbase = ((unsigned long long)bhi << 32) | blo;
+    int s = prev_start;
+#pragma unroll
+    for (int u = 0; u < COLUMN_PER_THREAD; ++u) {
+        const int i = i0 + u;
+        if (i < n) {
+            const int lo = __float_as_int(rb[u].x);
+            const unsigned long long cv = cbase + cl[u];
+            if (SE_STARTS(i, lo)) {
+                s = i;
+                cst[i] = cv;
+            }
+            if (i == n - 1 || ((los[i + 1] ^ lo) & 0xFF00FC) != 0) cen[s] = cv + col_one(lo);
+        }
+    }
+    __syncthreads();
+    s = prev_start;
+#pragma unroll
+    for (int u = 0; u < COLUMN_PER_THREAD; ++u) {
+        const int i = i0 + u;
+        if (i < n) {
+            const int lo = __float_as_int(rb[u].x);
+            if (SE_STARTS(i, lo)) s = i;
+            const int d = lo & 3;
+            const unsigned long long run = cen[s] - cst[s];
+            const int before = (d > 0 ? col_field(run, 0) : 0) + (d > 1 ? col_field(run, 1) : 0) +
+                               (d > 2 ? col_field(run, 2) : 0);
+            const int dst = s + before + col_field(cbase + cl[u] - cst[s], d);
+            if ((unsigned)dst < (unsigned)n) {   // always, by construction; keeps every store in the item
+                rp[2 * dst] = ra[u];
+                rp[2 * dst + 1] = rb[u];
+            }
+        }
+    }
+#undef SE_STARTS
+}
+
+void launch_column_order(Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items, int max_items,
+                         int dense_min, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_column_order, dim3(max_items), dim3(COLUMN_ORDER_THREADS), 0, s, rec, item_beg, item_end,
+                       n_items, dense_min);
+}
+
+// ---------------------------------------------------------------------------------
+// Column gather.  The y and x contractions are per particle and stay on the VALU:
+//
+//   U = sum_y Fy T,  V = sum_y Fy' T,  W = sum_y Fy T'   (T' with the z derivative factors)
+//   phi = sum_x Fx U,  F = (sum_x Fx' U, sum_x Fx V, sum_x Fx W)
+//
+// A group costs (rows spanned + 15) row steps of 4 (potential) or 8 (potential and
+// force) matrix instructions, whatever its 1..16 particles; in a dense column, sixteen
+// consecutive particles span one or two rows, so nearly every instruction is full.  Rows
+// outside a particle's window weigh exactly zero, so each particle's result depends only on
+// its own record and on the sub-grid, not on the grouping.
+//
 // v_mfma_f32_16x16x4f32 operand layouts (one block): A[i][k] in lane i + 16 k, B[k][j] in
 // lane j + 16 k, D[i][j] in register i mod 4 of lane j + 16 (i / 4).  Here lane j + 16 q
 // (q = 0..3) holds: A = sub-grid point (x0 + j, z0 + 4 q + c) of K step c; B = particle
 // j's z factor at window point 4 q + c; D = T_y[4 q + r][j] in register r.
+// ---------------------------------------------------------------------------------
 template <bool POT, bool FORCE>
 __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restrict__ rec,
                                                       const int* __restrict__ item_tile,
@@ -83,12 +223,11 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
-    // items below dense_min particles are the row gather's
-    if (n < dense_min) return;
+    // the other items are the row (or vector) gather's
+    if (!column_item(n, dense_min)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
-    int* slots = reinterpret_cast<int*>(sub + S2 * S + LDS_TILE_PAD);   // 16 record indices per wave
     const int tid = threadIdx.x, nthr = blockDim.x;
     {
         int tile = item_tile[item];
@@ -105,26 +244,26 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
     }
     __syncthreads();
 
+    // equal shares of the item's records, one per wave
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
-    const int col = wave & 3;                          // column of the share's first pencil
-    const int part = wave >> 2, nparts = nw >> 2;      // share of the item's records
-    const int rb = beg + (int)(((long)n * part) / nparts), re = beg + (int)(((long)n * (part + 1)) / nparts);
-    int* myslots = slots + 16 * wave;
+    const int rb = beg + (int)(((long)n * wave) / nw), re = beg + (int)(((long)n * (wave + 1)) / nw);
     const int j = lane & 15, q = lane >> 4;
     const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
     const float kc = -c * 1.4426950408889634f;
     const float fh = (float)g.first * h;
     const double h3 = g.h * g.h * g.h;
 
-    // One group of k particles, record indices in myslots[0 .. k).
-    auto process = [&](int k) {
-        __builtin_amdgcn_wave_barrier();
-        const int si = myslots[min(j, k - 1)];   // slots past k repeat the last particle
+    // Groups: runs of up to 16 consecutive records of one column; lane j + 16 q holds record
+    // g0 + j (slots past the run repeat its last record and store nothing).
+    for (int g0 = rb; g0 < re;) {
+        const int si = min(g0 + j, re - 1);
         const float4 r = *reinterpret_cast<const float4*>(rec + si);   // t[0], t[1], t[2], q
         const int lo = rec[si].lo, orig = rec[si].orig;
+        const int lo0 = __builtin_amdgcn_readfirstlane(lo);
+        const unsigned long long same = __ballot(q == 0 && g0 + j < re && (lo & 0xFF00FF) == (lo0 & 0xFF00FF));
+        const int k = __builtin_ctzll(~same);   // the run: lanes 0 .. k - 1 (1 <= k <= 16)
         const bool act = j < k;
         const int ly = (lo >> 8) & 255;
-        const int lo0 = __builtin_amdgcn_readfirstlane(lo);
         const int x0 = lo0 & 255, z0 = lo0 >> 16, y0 = (lo0 >> 8) & 255;
         const int y1 = ((__builtin_amdgcn_readlane(lo, k - 1) >> 8) & 255) + P;
         // B operands: particle j's z factors at window points 4 q + s, s = 0..3
@@ -139,7 +278,6 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
         const float tyf = r.y - fh;   // y offset of window index w: tyf - w h
         const v4f zero4 = {0.f, 0.f, 0.f, 0.f};
         v4f U = zero4, V = zero4, W = zero4;
-#pragma unroll 2
         for (int y = y0; y < y1; ++y) {
             const float* a = ap + y * S2;
             const float a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
@@ -194,88 +332,56 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
                 store_out(force, orig + (long)(q - 1) * N, qh * (double)sv);
             }
         }
-        __builtin_amdgcn_wave_barrier();
-    };
-
-    // Scan the share in sort order, 64 records at a time (the next batch's keys are loaded
-    // while the current one is processed).  A group takes particles of one pencil, at most
-    // 16, from one of its four columns: pencil p of the share (counted from 0 in sort order)
-    // gives this wave the column lx mod 4 == (col + p) mod 4, so that a pencil's dense and
-    // light columns fall to different waves from pencil to pencil.
-    int have = 0, gpen = -1, pcarry = 0, ppen = -1;
-    int lo_next = rb + lane < re ? rec[rb + lane].lo : -1;
-    for (int cb = rb; cb < re; cb += 64) {
-        const int i = cb + lane;
-        const int lo = lo_next;
-        lo_next = cb + 64 + lane < re ? rec[cb + 64 + lane].lo : -1;
-        const bool in = i < re;
-        const int pen = lo & 0xFF00FC;   // (lx / 4, lz): the pencil inside the tile
-        // pencil ordinal of each record: pencils started before this batch plus the pencil
-        // starts among lanes 0..lane of this batch
-        const int prev = __shfl_up(pen, 1, 64);
-        const bool start = in && pen != (lane == 0 ? ppen : prev);
-        const unsigned long long smask = __ballot(start);
-        const int pord = pcarry - 1 + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(smask >> 32),
-                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)smask, 0u))
-                         + (start ? 1 : 0);
-        pcarry += __builtin_popcountll(smask);
-        ppen = __builtin_amdgcn_readlane(pen, 63);
-        const bool match = in && (lo & 3) == ((col + pord) & 3);
-        unsigned long long m = __ballot(match);
-        while (m) {
-            const int pj = __builtin_amdgcn_readlane(pen, __builtin_ctzll(m));
-            if (have > 0 && pj != gpen) {
-                process(have);
-                have = 0;
-            }
-            gpen = pj;
-            // this pencil's remaining matches: a run, since the records are sorted by pencil
-            const unsigned long long same = __ballot(match && pen == gpen) & m;
-            const int take = min(__builtin_popcountll(same), 16 - have);
-            const unsigned long long taken = first_bits(same, take);
-            if ((taken >> lane) & 1ull) {
-                const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(taken >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo((unsigned)taken, 0u));
-                myslots[have + rank] = i;
-            }
-            have += take;
-            m &= ~taken;
-            if (have == 16) {
-                process(16);
-                have = 0;
-            }
-        }
+        g0 += k;
     }
-    if (have > 0) process(have);
+}
+
+// Column gather of the column items; block = 256, 512 or 1024.
+void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
+                         const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
+                         double* phi, double* force, const SysDesc* sys, int dense_min, int block, hipStream_t s)
+{
+    SE_REQUIRE(g.P == 16 && tl.S >= 16 + tl.TS - 1 && (block == 256 || block == 512 || block == 1024), SE_ERR_ARG,
+               "column gather: P = 16 and 256, 512 or 1024 threads");
+    const size_t lds = ((size_t)tl.S * tl.S * tl.S + LDS_TILE_PAD) * sizeof(float);
+    dim3 gr(tl.max_items), bl(block);
+#define SE_GC(POT, FORCE)                                                                                    \
+    do {                                                                                                     \
+        if (lds > 64 * 1024)                                                                                 \
+            (void)hipFuncSetAttribute((const void*)k_interp_col16<POT, FORCE>,                               \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                 \
+        hipLaunchKernelGGL((k_interp_col16<POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg, item_end, \
+                           n_items, grid, g, tl, N, phi, force, sys, dense_min);                             \
+    } while (0)
+    if (phi && force)
+        SE_GC(true, true);
+    else if (force)
+        SE_GC(false, true);
+    else
+        SE_GC(true, false);
+#undef SE_GC
 }
 
 // ---------------------------------------------------------------------------------
-// Column spread of the dense items (P = 16).  The same 1 x 1 columns: for a grid row y,
-// four particles k of one column add
+// Column spread.  For a grid row y, four particles k of one column add
 //
 //   D_y[x][z] += sum_k (q_k d Fx_k(x) Fy_k(y)) Fz_k(z),    x, z = 0..15
 //
-// which is one v_mfma_f32_16x16x4f32 (A = the particles' x factors times their y factor
-// at row y, B = their z factors; K = the four particles).  A wave keeps a ring of 20 grid
-// rows of its current column in registers (D of row y in ring slot y mod 20, four VGPRs
-// each).  A group is up to sixteen particles of one column whose windows lie in the ring
-// [y0, y0 + 20), y0 the first one's row: four K steps of four, each 20 matrix
+// one v_mfma_f32_16x16x4f32 (A = the particles' x factors times their y factor at row y,
+// B = their z factors; K = the four particles).  A wave keeps a ring of 20 grid rows of
+// its current column in registers (D of row y in ring slot y mod 20, four VGPRs each).  A
+// group is a run of up to sixteen consecutive records of one column whose windows lie in
+// the ring [y0, y0 + 20), y0 the first one's row: four K steps of four, each 20 matrix
 // instructions with no branch.  Lane i of particle k's 16-lane row holds the y factor of
 // ring slot i (and of slot 16 + i), so the A operand of slot s is one DPP multiply
 // (row_newbcast: s mod 16) with every register index static.  Rows that leave the ring are
 // added to the tile's FP64 LDS sub-grid: each value, scaled by 2^s per work item (q d 2^s
 // is folded into the x factor) and rounded to an integer, is added exactly, so the sub-grid
 // sums do not depend on the order of the atomics; the ring's own FP32 sums run in the fixed
-// order of the groups.  The sub-grid then becomes the item's FP32 partial, as in the pencil
-// spread.  Holding all S rows of the column instead, with a uniform branch per row and K
-// step, measured slower (clustered N = 4M at M = 256, dense items: 1.03 against 0.93 ms):
-// the branches cost more than the ring's extra rows.
-//
-// Work split: as in the column gather, four waves to a share of the item's records, each
-// taking one column of every pencil of the share (rotated from pencil to pencil).  A
-// group's particles are taken from the wave's current batch of 64 records (one record per
-// lane; the next batch is loaded meanwhile): their batch lanes go to a per-wave slot list in
-// LDS, and the lanes of each particle fetch its fields with ds_bpermute and form its factors.
+// order of the records.  The sub-grid then becomes the item's FP32 partial, as in the
+// pencil spread.  Holding all S rows of the column instead, with a uniform branch per row
+// and K step, measured slower (clustered N = 4M at M = 256, dense items: 1.03 against
+// 0.93 ms): the branches cost more than the ring's extra rows.
 // ---------------------------------------------------------------------------------
 constexpr int COL_RING = 20;
 constexpr int COL_KSTEPS = 4;   // K steps (of four particles) per group
@@ -319,15 +425,14 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
                                                       const int* __restrict__ n_items, Geom g, Tiling tl,
                                                       float* __restrict__ partials, int dense_min)
 {
-    constexpr int P = 16, GY = COL_RING, KS = COL_KSTEPS, GMAX = 4 * KS;
+    constexpr int P = 16, GY = COL_RING, KS = COL_KSTEPS;
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
-    // items below dense_min particles are the pencil and plane spreads'
-    if (n < dense_min) return;
+    // the other items are the pencil and plane spreads'
+    if (!column_item(n, dense_min)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ float wmax[12];
-    __shared__ int slots[12 * 16];   // per wave: the batch lanes of the open group's particles
     double* sub = reinterpret_cast<double*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -335,10 +440,9 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
     item_qmax_part(rec, beg, end, tid, nthr, wmax);
     __syncthreads();
 
+    // equal shares of the item's records, one per wave
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
-    const int col = wave & 3, part = wave >> 2, nparts = nw >> 2;
-    const int rb = beg + (int)(((long)n * part) / nparts), re = beg + (int)(((long)n * (part + 1)) / nparts);
-    int* myslots = slots + 16 * wave;
+    const int rb = beg + (int)(((long)n * wave) / nw), re = beg + (int)(((long)n * (wave + 1)) / nw);
     const int i16 = lane & 15, k4 = lane >> 4;   // lane = i16 + 16 k4: x / z index, particle slot
     // fixed-point scale of the item's LDS sums: q d 2^sc is folded into the x factor
     const int sc = __builtin_amdgcn_readfirstlane(flush_scale((double)n * item_qmax_read(wmax, nw) * g.d, g.d));
@@ -350,7 +454,7 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
     v4f acc[GY];
 #pragma unroll
     for (int s = 0; s < GY; ++s) acc[s] = v4f{0.f, 0.f, 0.f, 0.f};
-    int rpen = -1, base = 0, rofs = 0;   // ring: column key, first row, this lane's point (x0 + 4 k4, 0, z0 + i16)
+    int rcol = -1, base = 0, rofs = 0;   // ring: column, first row, this lane's point (x0 + 4 k4, 0, z0 + i16)
 
     // add ring rows [base, upto) (upto <= base + GY; rows past the sub-grid hold zeros)
     // into LDS and clear their slots
@@ -377,138 +481,58 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
         }
     };
 
-    // The open group: particle 4 cc + k4 of K step cc has its factors on this lane's row,
-    // formed when it joins the group: x factor (times q d 2^sc; zero on slots without a
-    // particle), z factor, y factors of ring slots i16 and 16 + (i16 & 3) for the ring
-    // starting at the group's first row.
-    float gqx[KS], gfz[KS], gy1[KS], gy2[KS];
-#pragma unroll
-    for (int cc = 0; cc < KS; ++cc) gqx[cc] = gfz[cc] = gy1[cc] = gy2[cc] = 0.f;
-
-    auto process = [&](int k, int gpen, int gy0, int lo0) __attribute__((always_inline)) {
+    // Groups: lanes 0 .. 15 read records g0 .. g0 + 15; the run of one column whose rows
+    // fit the ring from the first one's row is the group (1 to 16 particles, one pass of
+    // the loop each, so the ring has one flush site).  The lanes of particle 4 cc + k4
+    // fetch its fields by ds_bpermute and form its factors: x factor times q d 2^sc (zero
+    // on slots without a particle), z factor, y factors of ring slots i16 and
+    // 16 + (i16 & 3) of the ring [y0, y0 + GY).
+    for (int g0 = rb; g0 < re;) {
+        const int si = min(g0 + i16, re - 1);
+        const float4 ra = *reinterpret_cast<const float4*>(rec + si);   // t[0], t[1], t[2], q
+        const int lo = rec[si].lo;
+        const int lo0 = __builtin_amdgcn_readfirstlane(lo);
+        const int col = lo0 & 0xFF00FF, y0 = (lo0 >> 8) & 255;
+        const int ly = (lo >> 8) & 255;
+        const unsigned long long ok =
+            __ballot(k4 == 0 && g0 + i16 < re && (lo & 0xFF00FF) == col && ly <= y0 + (GY - P));
+        const int k = __builtin_ctzll(~ok);   // the group: lanes 0 .. k - 1 (1 <= k <= 16)
         // one flush site (rows [base, upto) of the old ring), so the ring registers are
         // never copied between control-flow paths
         int upto = base;
-        if (gpen != rpen) {
-            if (rpen >= 0) upto = base + GY;
-        } else if (gy0 > base) {
-            upto = min(gy0, base + GY);
+        if (col != rcol) {
+            if (rcol >= 0) upto = base + GY;
+        } else if (y0 > base) {
+            upto = min(y0, base + GY);
         }
         flush(upto);
-        if (gpen != rpen) {
-            rpen = gpen;
+        if (col != rcol) {
+            rcol = col;
             rofs = ((lo0 & 255) + 4 * k4) * S + (lo0 >> 16) + i16;
         }
-        base = gy0;
+        base = y0;
+        const int bm = y0 % GY;
+        const int t1 = i16 - bm, t2 = 16 + (i16 & 3) - bm;
+        const int yr1 = y0 + (t1 < 0 ? t1 + GY : t1), yr2 = y0 + (t2 < 0 ? t2 + GY : t2);   // rows of the slots
 #pragma unroll
         for (int cc = 0; cc < KS; ++cc) {
-            if (4 * cc < k) ColumnRing<0>::run(acc, gqx[cc], gy1[cc], gy2[cc], gfz[cc]);
-            gqx[cc] = 0.f;   // the next group starts with empty slots
+            if (4 * cc < k) {
+                const int p = 4 * cc + k4;
+                const int sl = 4 * min(p, k - 1);
+                const float t0 = bperm_f(sl, ra.x), ty = bperm_f(sl, ra.y), tz = bperm_f(sl, ra.z),
+                            qv = bperm_f(sl, ra.w);
+                const int lyp = __builtin_amdgcn_ds_bpermute(sl, ly);
+                const int w1 = yr1 - lyp, w2 = yr2 - lyp;   // window indices of the two slots' rows
+                const float qx = p < k ? qv * dn * gauss_c(kc, t0 - oi) : 0.f;
+                const float fz = gauss_c(kc, tz - oi);
+                const float f1 = (unsigned)w1 < (unsigned)P ? gauss_c(kc, ty - (float)(g.first + w1) * h) : 0.f;
+                const float f2 = (unsigned)w2 < (unsigned)P ? gauss_c(kc, ty - (float)(g.first + w2) * h) : 0.f;
+                ColumnRing<0>::run(acc, qx, f1, f2, fz);
+            }
         }
-    };
-
-    // Scan (as in the column gather); the next batch of records is loaded while the current
-    // one is processed.  Each pass of the outer loop fills one group and processes it at a
-    // single call site.
-    int have = 0, gpen = -1, gy0 = 0, glo0 = 0, pcarry = 0, ppen = -1;
-    if (lane < 16) myslots[lane] = 0;   // every slot names a lane before its first fill
-    int cb = rb - 64;                    // current batch (none yet)
-    float4 ra = {0.f, 0.f, 0.f, 0.f}, ra_next = {0.f, 0.f, 0.f, 0.f};
-    int lo = -1, lo_next = -1, pen = 0, ly = 0;
-    bool match = false;
-    unsigned long long m = 0;
-    if (rb + lane < re) {
-        ra_next = *reinterpret_cast<const float4*>(rec + rb + lane);
-        lo_next = rec[rb + lane].lo;
+        g0 += k;
     }
-    for (;;) {
-        bool close = false;
-        while (!close) {
-            if (m == 0) {
-                if (cb + 64 >= re) break;   // no records left
-                cb += 64;
-                const int i = cb + lane;
-                ra = ra_next;
-                lo = lo_next;
-                lo_next = -1;
-                if (cb + 64 + lane < re) {
-                    ra_next = *reinterpret_cast<const float4*>(rec + cb + 64 + lane);
-                    lo_next = rec[cb + 64 + lane].lo;
-                }
-                const bool in = i < re;
-                pen = lo & 0xFF00FC;
-                ly = (lo >> 8) & 255;
-                const int prev = __shfl_up(pen, 1, 64);
-                const bool start = in && pen != (lane == 0 ? ppen : prev);
-                const unsigned long long smask = __ballot(start);
-                const int pord = pcarry - 1 +
-                                 (int)__builtin_amdgcn_mbcnt_hi((unsigned)(smask >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo((unsigned)smask, 0u)) +
-                                 (start ? 1 : 0);
-                pcarry += __builtin_popcountll(smask);
-                ppen = __builtin_amdgcn_readlane(pen, 63);
-                match = in && (lo & 3) == ((col + pord) & 3);
-                m = __ballot(match);
-                continue;
-            }
-            const int j = __builtin_ctzll(m);
-            const int pj = __builtin_amdgcn_readlane(pen, j), lyj = __builtin_amdgcn_readlane(ly, j);
-            if (have > 0 && (pj != gpen || lyj > gy0 + (GY - P))) break;   // the group is complete
-            if (have == 0) {
-                gpen = pj;
-                gy0 = lyj;
-                glo0 = __builtin_amdgcn_readlane(lo, j);
-            }
-            // this pencil's next matches whose windows fit the ring from gy0 (a run: the
-            // records are sorted by pencil, then row)
-            const unsigned long long same = __ballot(match && pen == gpen && ly <= gy0 + (GY - P)) & m;
-            const int take = min(__builtin_popcountll(same), GMAX - have);
-            const unsigned long long taken = first_bits(same, take);
-            if ((taken >> lane) & 1ull) {
-                const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(taken >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo((unsigned)taken, 0u));
-                myslots[have + rank] = lane;
-            }
-            __builtin_amdgcn_wave_barrier();
-            // the lanes of particle 4 cc + k4 fetch its fields from the batch lane in its slot
-            // and form its factors; the ring will start at row gy0 (slot s holds the row
-            // == s (mod GY) of [gy0, gy0 + GY))
-            const int bm = gy0 % GY;
-            const int t1 = i16 - bm, t2 = 16 + (i16 & 3) - bm;
-            const int y1 = gy0 + (t1 < 0 ? t1 + GY : t1), y2 = gy0 + (t2 < 0 ? t2 + GY : t2);
-#pragma unroll
-            for (int cc = 0; cc < KS; ++cc) {
-                if (4 * cc + 3 >= have && 4 * cc < have + take) {   // K step cc gains particles
-                    const int p = 4 * cc + k4;
-                    const int sl = 4 * myslots[p];
-                    const bool fill = p >= have && p < have + take;
-                    const float t0 = __int_as_float(__builtin_amdgcn_ds_bpermute(sl, __float_as_int(ra.x)));
-                    const float ty = __int_as_float(__builtin_amdgcn_ds_bpermute(sl, __float_as_int(ra.y)));
-                    const float t2f = __int_as_float(__builtin_amdgcn_ds_bpermute(sl, __float_as_int(ra.z)));
-                    const float q = __int_as_float(__builtin_amdgcn_ds_bpermute(sl, __float_as_int(ra.w)));
-                    const int lyp = __builtin_amdgcn_ds_bpermute(sl, ly);
-                    const int w1 = y1 - lyp, w2 = y2 - lyp;   // window indices of the two slots' rows
-                    const float qx = q * dn * gauss_c(kc, t0 - oi), fz = gauss_c(kc, t2f - oi);
-                    const float f1 = (unsigned)w1 < (unsigned)P ? gauss_c(kc, ty - (float)(g.first + w1) * h) : 0.f;
-                    const float f2 = (unsigned)w2 < (unsigned)P ? gauss_c(kc, ty - (float)(g.first + w2) * h) : 0.f;
-                    if (fill) {
-                        gqx[cc] = qx;
-                        gfz[cc] = fz;
-                        gy1[cc] = f1;
-                        gy2[cc] = f2;
-                    }
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            have += take;
-            m &= ~taken;
-            close = have == GMAX;
-        }
-        if (have == 0) break;
-        process(have, gpen, gy0, glo0);
-        have = 0;
-    }
-    if (rpen >= 0) flush(base + GY);
+    if (rcol >= 0) flush(base + GY);
     __syncthreads();
     float* out = partials + (size_t)item * tl.part_elems;
     SubWalk w(S, tid, nthr);   // canonical [x][y][z]: a = ix, b = iy, c = iz
@@ -518,9 +542,9 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
     }
 }
 
-// Column spread of the items of at least dense_min particles: work-groups of `block`
-// threads (256, 512 or 768; the kernel is held to 168 registers, 3 waves per SIMD), one
-// per CU (the FP64 sub-grid fills the LDS).
+// Column spread of the column items: work-groups of `block` threads (256, 512 or 768; the
+// kernel is held to 168 registers, 3 waves per SIMD), one per CU (the FP64 sub-grid fills
+// the LDS).
 bool launch_spread_col16(const Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items,
                          const Geom& g, const Tiling& tl, float* partials, int dense_min, int block, hipStream_t s)
 {
@@ -532,32 +556,6 @@ bool launch_spread_col16(const Rec<float>* rec, const int* item_beg, const int* 
     hipLaunchKernelGGL(k_spread_col16, dim3(tl.max_items), dim3(block), lds, s, rec, item_beg, item_end, n_items, g,
                        tl, partials, dense_min);
     return true;
-}
-
-// Column gather of the items of at least dense_min particles; block = 256, 512 or 1024.
-void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
-                         const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
-                         double* phi, double* force, const SysDesc* sys, int dense_min, int block, hipStream_t s)
-{
-    SE_REQUIRE(g.P == 16 && tl.S >= 16 + tl.TS - 1 && (block == 256 || block == 512 || block == 1024), SE_ERR_ARG,
-               "column gather: P = 16 and 256, 512 or 1024 threads");
-    const size_t lds = ((size_t)tl.S * tl.S * tl.S + LDS_TILE_PAD) * sizeof(float) + 16 * (block / 64) * sizeof(int);
-    dim3 gr(tl.max_items), bl(block);
-#define SE_GC(POT, FORCE)                                                                                    \
-    do {                                                                                                     \
-        if (lds > 64 * 1024)                                                                                 \
-            (void)hipFuncSetAttribute((const void*)k_interp_col16<POT, FORCE>,                               \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                 \
-        hipLaunchKernelGGL((k_interp_col16<POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg, item_end, \
-                           n_items, grid, g, tl, N, phi, force, sys, dense_min);                             \
-    } while (0)
-    if (phi && force)
-        SE_GC(true, true);
-    else if (force)
-        SE_GC(false, true);
-    else
-        SE_GC(true, false);
-#undef SE_GC
 }
 
 }  // namespace se

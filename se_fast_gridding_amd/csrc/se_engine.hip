@@ -501,10 +501,16 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int
         launch_scatter_group<T>(pl->d_sys, S, blocks, pl->g, tl, pos, (Rec<T>*)pl->rec, s);
     else
         launch_scatter<T>(x, q, N, pl->g, tl, pos, (Rec<T>*)pl->rec, s);
-    pl->rec_ev(1);
     // work items: at most one partial item per non-empty tile plus one per full chunk
     Tiling tlc = tl;
     tlc.max_items = std::min(tl.max_items, nt + N / tl.chunk + 1);
+    // the column items' records into column order, for the column spread and gather
+    if constexpr (sizeof(T) == 4) {
+        if (tl.column)
+            launch_column_order((Rec<float>*)pl->rec, pl->item_beg, pl->item_end, pl->n_items, tlc.max_items,
+                                column_dense_min(tl), s);
+    }
+    pl->rec_ev(1);
     bool done = false;
     if constexpr (sizeof(T) == 4) {
         if (tl.pencil)

@@ -345,13 +345,27 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
                           const Geom& g, const Tiling& tl, int N, double* phi, double* force,
                           const SysDesc* sys, hipStream_t s, hipStream_t side = nullptr,
                           hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
-// column gather (se_column.hip): items of at least dense_min particles, P = 16 FP32,
-// work-groups of 256 or 512 threads
+// Column items (the dense work items of the P = 16 FP32 path with matrix cores): dense_min
+// to COLUMN_MAX particles.  k_column_order puts their records in column order (staged in
+// LDS, so their number is capped); the column spread and gather take them and every other
+// spread and gather kernel skips them (dense_min 0: no column items).
+constexpr int COLUMN_MAX = 2048;
+__host__ __device__ inline bool column_item(int n, int dense_min)
+{
+    return dense_min > 0 && n >= dense_min && n <= COLUMN_MAX;
+}
+// the column items' threshold: 2.5 particles per (pencil, row) key
+inline int column_dense_min(const Tiling& tl) { return 5 * tl.K / 2; }
+// records of the column items into column order (pencil, lx mod 4, row), in place
+void launch_column_order(Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items, int max_items,
+                         int dense_min, hipStream_t s);
+// column gather (se_column.hip): column items, P = 16 FP32, work-groups of 256, 512 or
+// 1024 threads
 void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                          const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
                          double* phi, double* force, const SysDesc* sys, int dense_min, int block, hipStream_t s);
-// column spread (se_column.hip): items of at least dense_min particles, P = 16 FP32,
-// work-groups of 256, 512 or 768 threads; false if the sub-grid side has no instance
+// column spread (se_column.hip): column items, P = 16 FP32, work-groups of 256, 512 or
+// 768 threads
 bool launch_spread_col16(const Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items,
                          const Geom& g, const Tiling& tl, float* partials, int dense_min, int block, hipStream_t s);
 template <typename T>

@@ -271,10 +271,9 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
     const int item = blockIdx.x;
     if (item >= *n_items) return;   // uniform across the work-group
     // sparse items (whole tiles of fewer than sparse_max particles) are the plane
-    // spread's (sparse_max > 0), items of dense_min particles or more the column spread's
-    // (dense_min > 0)
+    // spread's (sparse_max > 0), the column items the column spread's
     if (sparse_item(item_tile, item_beg, item_end, tile_start, item, sparse_max)) return;
-    if (dense_min > 0 && item_end[item] - item_beg[item] >= dense_min) return;
+    if (column_item(item_end[item] - item_beg[item], dense_min)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ float wmax[spread_pencil_max_threads(P) / 64];
     double* sub = reinterpret_cast<double*>(smem_raw);   // [y][x][z], S^3
@@ -556,8 +555,8 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
-    // items of dense_min particles or more are the MFMA gather's (dense_min > 0)
-    if (dense_min > 0 && n >= dense_min) return;
+    // the column items are the matrix-core gather's
+    if (column_item(n, dense_min)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -803,8 +802,8 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
-    // items below dense_min particles are left to the vector gather
-    if (n < dense_min) return;
+    // the column items only; the others are the row or vector gather's
+    if (!column_item(n, dense_min)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -1045,8 +1044,8 @@ __global__ __launch_bounds__(512, 4) void k_interp_row16(const Rec<float>* __res
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
-    // items of dense_min particles or more are the MFMA gather's
-    if (n >= dense_min) return;
+    // the column items are the matrix-core gather's
+    if (column_item(n, dense_min)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], SS^3
     const int M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -1147,9 +1146,8 @@ static void allow_lds_p(const void* fn, size_t bytes)
 // the pencil spread; 0 turns the plane spread off.
 int spread_sparse_max(const Tiling& tl) { return tl.plane ? sparse_tile_max(tl) : 0; }
 
-// Items of this many particles or more take the column spread (0: none); the gather's
-// threshold for its column kernel.
-static int spread_dense_min(const Tiling& tl, int P) { return (tl.column && P == 16) ? 5 * tl.K / 2 : 0; }
+// Threshold of the column items the column spread takes (0: none).
+static int spread_dense_min(const Tiling& tl, int P) { return (tl.column && P == 16) ? column_dense_min(tl) : 0; }
 
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const int* tile_start, const Geom& g, const Tiling& tl, float* partials,
@@ -1159,9 +1157,8 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int
     SE_REQUIRE(tl.sblock >= 64 && tl.sblock % 64 == 0 && tl.sblock <= spread_pencil_max_threads(g.P), SE_ERR_ARG,
                "spread work-group size past the kernel's bound");
     const int sparse_max = spread_sparse_max(tl);
-    int dense_min = spread_dense_min(tl, g.P);
-    if (dense_min > 0 && !launch_spread_col16(rec, item_beg, item_end, n_items, g, tl, partials, dense_min, 768, s))
-        dense_min = 0;   // no column spread for this sub-grid side: the pencil spread takes every item
+    const int dense_min = spread_dense_min(tl, g.P);
+    if (dense_min > 0) launch_spread_col16(rec, item_beg, item_end, n_items, g, tl, partials, dense_min, 768, s);
     dim3 gr(tl.max_items), bl(tl.sblock);
 #define SE_SP(PV)                                                                                            \
     do {                                                                                                     \
@@ -1238,6 +1235,7 @@ static bool row16_supported(const Tiling& tl) { return tl.S == 19 || tl.S == 23 
 int gather_dense_min(const Tiling& tl)
 {
     if (!tl.mfma) return 0;
+    if (tl.column) return column_dense_min(tl);   // the items put in column order
     return row16_supported(tl) ? 5 * tl.K / 2 : sparse_tile_max(tl);
 }
 
