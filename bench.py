@@ -217,7 +217,7 @@ def algorithmic(N, P, M, force=True, esize=4):
 
 
 def measure(N, M, P, xi, gen, wdesc, prec_name, steps, warmup, check, tile=0, chunk=0, block=0, generic=False,
-            mfma=True):
+            mfma=True, column=True):
     """One plan, `warmup` + `steps` full solves timed by the wall clock, then `steps` solves
     with HIP events on the plan's stream around each stage; FP32 accuracy against the FP64
     P = 21 solve of the same system (FGG floor about 1e-13)."""
@@ -234,7 +234,7 @@ def measure(N, M, P, xi, gen, wdesc, prec_name, steps, warmup, check, tile=0, ch
     prec = se.PREC_FP32 if prec_name == "fp32" else se.PREC_FP64
     stream = torch.cuda.current_stream()
     plan = se.Plan(M, P, 1.0, xi, N, precision=prec, stream=stream, tile=tile, chunk=chunk, block=block,
-                   generic=generic, mfma=mfma)
+                   generic=generic, mfma=mfma, column=column)
     for _ in range(warmup):
         plan.solve(xt, qt, phi, F)
     torch.cuda.synchronize()
@@ -313,7 +313,7 @@ def run(args):
     wN, wM, wP, wxi, gen, wdesc = WORKLOADS[args.workload]
     N, M, P, xi = args.n or wN, args.m or wM, args.p or wP, args.xi or wxi
     r = measure(N, M, P, xi, gen, wdesc, args.precision, args.steps, args.warmup, args.check, args.tile,
-                args.chunk, args.block, args.generic, args.mfma)
+                args.chunk, args.block, args.generic, args.mfma, args.column)
     rl = roofline(r, args.workload, args.precision)
     extras = {}
     if args.extras and args.workload == "clustered" and not (args.n or args.m or args.p or args.xi):
@@ -622,6 +622,8 @@ def main():
                     help="batched: skip the tiling autotuner (se_tune) before the timed runs")
     ap.add_argument("--no-mfma", dest="mfma", action="store_false",
                     help="P = 16 FP32 gather on the vector register-ring kernel instead of the matrix cores")
+    ap.add_argument("--no-column", dest="column", action="store_false",
+                    help="P = 16 FP32 dense items on the 4 x 1 pencil kernels instead of the 1 x 1 column kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has exactly one; "

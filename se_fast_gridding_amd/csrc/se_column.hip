@@ -254,11 +254,15 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
     const double h3 = g.h * g.h * g.h;
 
     // Groups: runs of up to 16 consecutive records of one column; lane j + 16 q holds record
-    // g0 + j (slots past the run repeat its last record and store nothing).
+    // g0 + j (slots past the run repeat its last record and store nothing).  The next
+    // group's records are loaded as soon as its first record is known, so their latency
+    // runs under this group's rows.
+    if (rb >= re) return;
+    v4f rn = *reinterpret_cast<const v4f*>(rec + min(rb + j, re - 1));   // t[0], t[1], t[2], q
+    int lon = rec[min(rb + j, re - 1)].lo, orn = rec[min(rb + j, re - 1)].orig;
     for (int g0 = rb; g0 < re;) {
-        const int si = min(g0 + j, re - 1);
-        const float4 r = *reinterpret_cast<const float4*>(rec + si);   // t[0], t[1], t[2], q
-        const int lo = rec[si].lo, orig = rec[si].orig;
+        const v4f r = rn;
+        const int lo = lon, orig = orn;
         const int lo0 = __builtin_amdgcn_readfirstlane(lo);
         const unsigned long long same = __ballot(q == 0 && g0 + j < re && (lo & 0xFF00FF) == (lo0 & 0xFF00FF));
         const int k = __builtin_ctzll(~same);   // the run: lanes 0 .. k - 1 (1 <= k <= 16)
@@ -266,6 +270,12 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
         const int ly = (lo >> 8) & 255;
         const int x0 = lo0 & 255, z0 = lo0 >> 16, y0 = (lo0 >> 8) & 255;
         const int y1 = ((__builtin_amdgcn_readlane(lo, k - 1) >> 8) & 255) + P;
+        {
+            const int sn = min(g0 + k + j, re - 1);   // the next group's records
+            rn = *reinterpret_cast<const v4f*>(rec + sn);
+            lon = rec[sn].lo;
+            orn = rec[sn].orig;
+        }
         // B operands: particle j's z factors at window points 4 q + s, s = 0..3
         float bz[4], bzd[4];
 #pragma unroll
@@ -278,9 +288,15 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
         const float tyf = r.y - fh;   // y offset of window index w: tyf - w h
         const v4f zero4 = {0.f, 0.f, 0.f, 0.f};
         v4f U = zero4, V = zero4, W = zero4;
+        // Software pipeline: the next row's sub-grid values are read, and the previous
+        // row's y sums formed, around this row's matrix instructions, so neither the LDS
+        // latency nor the matrix results stall the wave.  After the last row the read is of
+        // row S - 1 again (in the sub-grid) and is not used.
+        v4f Tp = zero4, Tdp = zero4;
+        float fyp = 0.f, fydp = 0.f;
+        const float* a = ap + y0 * S2;
+        float a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
         for (int y = y0; y < y1; ++y) {
-            const float* a = ap + y * S2;
-            const float a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
             v4f T = mfma16(a0, bz[0], zero4);
             T = mfma16(a1, bz[1], T);
             T = mfma16(a2, bz[2], T);
@@ -292,17 +308,32 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
                 Td = mfma16(a2, bzd[2], Td);
                 Td = mfma16(a3, bzd[3], Td);
             }
-            const int jy = y - ly;
-            const float ay = fmaf(-(float)jy, h, tyf);
-            const float fy = (act && (unsigned)jy < (unsigned)P) ? gauss_c(kc, ay) : 0.f;
-            const float fyd = fy * (-c * ay);
+            a = ap + min(y + 1, S - 1) * S2;
+            a0 = a[0];
+            a1 = a[1];
+            a2 = a[2];
+            a3 = a[3];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                U[i] = fmaf(fy, T[i], U[i]);
+                U[i] = fmaf(fyp, Tp[i], U[i]);
                 if (FORCE) {
-                    V[i] = fmaf(fyd, T[i], V[i]);
-                    W[i] = fmaf(fy, Td[i], W[i]);
+                    V[i] = fmaf(fydp, Tp[i], V[i]);
+                    W[i] = fmaf(fyp, Tdp[i], W[i]);
                 }
+            }
+            const int jy = y - ly;
+            const float ay = fmaf(-(float)jy, h, tyf);
+            fyp = (act && (unsigned)jy < (unsigned)P) ? gauss_c(kc, ay) : 0.f;
+            fydp = fyp * (-c * ay);
+            Tp = T;
+            Tdp = Td;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            U[i] = fmaf(fyp, Tp[i], U[i]);
+            if (FORCE) {
+                V[i] = fmaf(fydp, Tp[i], V[i]);
+                W[i] = fmaf(fyp, Tdp[i], W[i]);
             }
         }
         // x contraction over this lane's rows x = 4 q + i, then over the four lane rows
@@ -487,16 +518,28 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
     // fetch its fields by ds_bpermute and form its factors: x factor times q d 2^sc (zero
     // on slots without a particle), z factor, y factors of ring slots i16 and
     // 16 + (i16 & 3) of the ring [y0, y0 + GY).
+    // The next group's records are loaded as soon as its first record is known (a wave
+    // with an empty share loads nothing; it still takes the barrier below).
+    v4f rn = {0.f, 0.f, 0.f, 0.f};   // t[0], t[1], t[2], q
+    int lon = 0;
+    if (rb < re) {
+        rn = *reinterpret_cast<const v4f*>(rec + min(rb + i16, re - 1));
+        lon = rec[min(rb + i16, re - 1)].lo;
+    }
     for (int g0 = rb; g0 < re;) {
-        const int si = min(g0 + i16, re - 1);
-        const float4 ra = *reinterpret_cast<const float4*>(rec + si);   // t[0], t[1], t[2], q
-        const int lo = rec[si].lo;
+        const v4f ra = rn;
+        const int lo = lon;
         const int lo0 = __builtin_amdgcn_readfirstlane(lo);
         const int col = lo0 & 0xFF00FF, y0 = (lo0 >> 8) & 255;
         const int ly = (lo >> 8) & 255;
         const unsigned long long ok =
             __ballot(k4 == 0 && g0 + i16 < re && (lo & 0xFF00FF) == col && ly <= y0 + (GY - P));
         const int k = __builtin_ctzll(~ok);   // the group: lanes 0 .. k - 1 (1 <= k <= 16)
+        {
+            const int sn = min(g0 + k + i16, re - 1);   // the next group's records
+            rn = *reinterpret_cast<const v4f*>(rec + sn);
+            lon = rec[sn].lo;
+        }
         // one flush site (rows [base, upto) of the old ring), so the ring registers are
         // never copied between control-flow paths
         int upto = base;
