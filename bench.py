@@ -41,31 +41,35 @@ HBM_PEAK_GBS = 8000.0
 #   valu_busy_pct = rocprofv3 VALUBusy = 100 SQ_ACTIVE_INST_VALU / CU_NUM / GRBM_GUI_ACTIVE
 #   mfma_util_pct = rocprofv3 MfmaUtil = 100 SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE SIMD_NUM)
 #   fma_insts     = SQ_INSTS_VALU_FMA_F32 (wave instructions, 64 lanes each) of all VALU
-#                   instructions SQ_INSTS_VALU
-# The spread and the gather of the clustered workload are two kernels each (spread: pencil
-# register ring on dense items, plane spread on sparse tiles, one after the other; gather:
-# matrix-core gather on dense items, vector gather on sparse items, concurrent on two
-# streams); each is listed under its stage.
+#                   instructions SQ_INSTS_VALU (which counts the MFMAs, SQ_INSTS_MFMA, too)
+# The spread and the gather of the clustered workload are several kernels each (spread:
+# column spread on the dense items, pencil register ring on the medium ones, plane spread on
+# the sparse tiles, one after the other; gather: column gather on the dense items and row
+# gather on the others, concurrently on two streams); each is listed under its stage.
 MEASURED = {
     ("clustered", 4_000_000, 16, 256, "fp32", 12): {
-        "spread": {"traffic_bytes": 251.4e6 + 831.8e6,
-                   "k_spread_pencil": {"traffic_bytes": 251.4e6, "valu_busy_pct": 47.9, "valu_insts": 4.062e8,
-                                       "fma_insts": 2.646e8, "lds_bank_conflicts": 1.13e6, "lds_insts": 2.758e7},
-                   "k_spread_plane": {"traffic_bytes": 831.8e6, "valu_busy_pct": 78.2, "valu_insts": 2.639e8,
-                                      "fma_insts": 1.784e8, "lds_bank_conflicts": 7.70e5, "lds_insts": 2.133e7}},
-        "gather": {"traffic_bytes": 670.6e6 + 1240.7e6,
-                   "k_interp_mfma16": {"traffic_bytes": 670.6e6, "mfma_util_pct": 32.4, "valu_busy_pct": 36.4,
-                                       "mfma_insts": 1.633e8, "lds_bank_conflicts": 5.25e6, "lds_insts": 4.019e7},
-                   "k_interp_pencil": {"traffic_bytes": 1240.7e6, "valu_busy_pct": 63.2,
-                                       "lds_bank_conflicts": 6.36e6, "lds_insts": 3.997e7}},
+        "spread": {"traffic_bytes": 285.3e6 + 41.6e6 + 832.0e6,
+                   "k_spread_col16": {"traffic_bytes": 285.3e6, "valu_busy_pct": 22.1, "mfma_util_pct": 25.8,
+                                      "valu_insts": 1.04e8, "mfma_insts": 1.57e7, "lds_bank_conflicts": 1.45e6,
+                                      "lds_insts": 1.07e7},
+                   "k_spread_pencil": {"traffic_bytes": 41.6e6, "valu_busy_pct": 19.2, "valu_insts": 3.22e7,
+                                       "fma_insts": 1.67e7, "lds_bank_conflicts": 2.48e5, "lds_insts": 2.94e6},
+                   "k_spread_plane": {"traffic_bytes": 832.0e6, "valu_busy_pct": 79.5, "valu_insts": 2.64e8,
+                                      "fma_insts": 1.78e8, "lds_bank_conflicts": 7.7e5, "lds_insts": 2.13e7}},
+        "gather": {"traffic_bytes": 646.5e6 + 1281.9e6,
+                   "k_interp_col16": {"traffic_bytes": 646.5e6, "mfma_util_pct": 37.6, "valu_busy_pct": 30.2,
+                                      "valu_insts": 1.76e8, "mfma_insts": 2.84e7, "fma_insts": 5.39e7,
+                                      "lds_bank_conflicts": 2.84e7, "lds_insts": 7.11e6},
+                   "k_interp_row16": {"traffic_bytes": 1281.9e6, "valu_busy_pct": 70.8, "valu_insts": 3.0e8,
+                                      "fma_insts": 1.4e8, "lds_bank_conflicts": 1.12e8, "lds_insts": 3.18e7}},
     },
     ("uniform", 1_000_000, 16, 128, "fp32", 12): {
         "spread": {"traffic_bytes": 137.3e6,
-                   "k_spread_pencil": {"traffic_bytes": 137.3e6, "valu_busy_pct": 64.9, "valu_insts": 1.57e8,
-                                       "fma_insts": 8.206e7, "lds_bank_conflicts": 8.19e5, "lds_insts": 1.514e7}},
-        "gather": {"traffic_bytes": 273.6e6,
-                   "k_interp_mfma16": {"traffic_bytes": 273.6e6, "mfma_util_pct": 32.7, "valu_busy_pct": 41.0,
-                                       "mfma_insts": 5.377e7, "lds_bank_conflicts": 1.406e7, "lds_insts": 1.94e7}},
+                   "k_spread_pencil": {"traffic_bytes": 137.3e6, "valu_busy_pct": 64.5, "valu_insts": 1.57e8,
+                                       "fma_insts": 8.21e7, "lds_bank_conflicts": 8.19e5, "lds_insts": 1.51e7}},
+        "gather": {"traffic_bytes": 280.9e6,
+                   "k_interp_row16": {"traffic_bytes": 280.9e6, "valu_busy_pct": 70.1, "valu_insts": 2.05e8,
+                                      "fma_insts": 1.41e8, "lds_bank_conflicts": 1.1e8, "lds_insts": 3.21e7}},
     },
 }
 
@@ -289,16 +293,18 @@ def roofline(r, workload, prec_name):
     upd = float(N) * P ** 3
     kernels = {}
     meas_all = MEASURED.get((workload, N, P, M, prec_name, tile), {})
-    for name, key, fl, kms in (("spread (k_spread_pencil + k_spread_plane)", "spread", 2.0 * upd, ms["spread_kernel"]),
-                               ("gather (k_interp_mfma16 + k_interp_pencil)", "gather", 4.0 * upd,
+    for name, key, fl, kms in (("spread (k_spread_col16 + k_spread_pencil + k_spread_plane)", "spread", 2.0 * upd,
+                                ms["spread_kernel"]),
+                               ("gather (k_interp_col16 + k_interp_row16)", "gather", 4.0 * upd,
                                 ms["gather_kernel"])):
         meas = meas_all.get(key, {})
         kernels[name] = {"ms": kms, "flops": fl, "achieved_tflops": fl / (kms * 1e-3) / 1e12,
                          "frac": fl / (kms * 1e-3) / 1e12 / peak, "counters": meas or None}
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     return {"bound": "mfma",
-            "compute": ("FP32: vector FMA (spread, vector gather) and FP32 matrix cores v_mfma_f32_4x4x1_16b_f32 "
-                        "(dense gather); peak is MI355X dense FP32, 157.3 TFLOP/s, the same for both units"),
+            "compute": ("FP32: matrix cores v_mfma_f32_16x16x4f32 (column spread and gather of the dense items) and "
+                        "vector FMA (pencil, plane and row kernels of the others); peak is MI355X dense FP32, "
+                        "157.3 TFLOP/s, the same for both units (on this GPU they do not run at once)"),
             "achieved": flops / t_meas / 1e12, "peak": peak, "unit": "TFLOP/s",
             "frac": t_roof / t_meas, "t_roof_ms": t_roof * 1e3, "t_measured_ms": t_meas * 1e3,
             "flops": flops, "bytes_algorithmic": bytes_,

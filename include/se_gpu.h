@@ -56,7 +56,8 @@ typedef struct {
     int tile;          /* tile side in grid cells (at most 16), 0 = automatic */
     int chunk;         /* max particles per work item, 0 = automatic */
     int block;         /* threads per work-group (<= 512) of every kernel (the P = 16 MFMA gather
-                          uses at most 256); 0 = each kernel's own default: 768 for the FP32
+                          uses at most 256; the P = 16 column spread and gather always run
+                          768 and 512); 0 = each kernel's own default: 768 for the FP32
                           register-ring spread, 256 for the MFMA gather, 512 otherwise */
     int flags;         /* SE_PLAN_* bits */
     double slab;       /* SE2P: particles have 0 <= z < slab; SE1P: particles have
