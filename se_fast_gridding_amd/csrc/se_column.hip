@@ -406,13 +406,17 @@ void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int*
 // instructions with no branch.  Lane i of particle k's 16-lane row holds the y factor of
 // ring slot i (and of slot 16 + i), so the A operand of slot s is one DPP multiply
 // (row_newbcast: s mod 16) with every register index static.  Rows that leave the ring are
-// added to the tile's FP64 LDS sub-grid: each value, scaled by 2^s per work item (q d 2^s
-// is folded into the x factor) and rounded to an integer, is added exactly, so the sub-grid
+// added to the tile's LDS sub-grid: each value, scaled by 2^s per work item (q d 2^s is
+// folded into the x factor) and rounded to an integer, is added exactly, so the sub-grid
 // sums do not depend on the order of the atomics; the ring's own FP32 sums run in the fixed
 // order of the records.  The sub-grid then becomes the item's FP32 partial, as in the
-// pencil spread.  Holding all S rows of the column instead, with a uniform branch per row
-// and K step, measured slower (clustered N = 4M at M = 256, dense items: 1.03 against
-// 0.93 ms): the branches cost more than the ring's extra rows.
+// pencil spread, except that the sub-grid holds 32-bit integers (quantum about
+// n max|q| d 2^-29 per item): half the FP64 sub-grid's LDS, so two work-groups share a CU
+// and one's zeroing, flush and write-out overlap the other's matrix work, and
+// `ds_add_u32` runs at about twice the rate of `ds_add_f64`.  Holding all S rows of the
+// column instead of the ring, with a uniform branch per row and K step, measured slower
+// (clustered N = 4M at M = 256, dense items: 1.03 against 0.93 ms): the branches cost
+// more than the ring's extra rows.
 // ---------------------------------------------------------------------------------
 constexpr int COL_RING = 20;
 constexpr int COL_KSTEPS = 4;   // K steps (of four particles) per group
@@ -464,10 +468,10 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
     if (!column_item(n, dense_min)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ float wmax[12];
-    double* sub = reinterpret_cast<double*>(smem_raw);   // [y][x][z], S^3
+    int* sub = reinterpret_cast<int*>(smem_raw);   // [y][x][z], S^3 fixed-point sums
     const int S = tl.S, S2 = S * S, S3 = S2 * S;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int e = tid; e < S3; e += nthr) sub[e] = 0.0;
+    for (int e = tid; e < S3; e += nthr) sub[e] = 0;
     item_qmax_part(rec, beg, end, tid, nthr, wmax);
     __syncthreads();
 
@@ -476,7 +480,7 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
     const int rb = beg + (int)(((long)n * wave) / nw), re = beg + (int)(((long)n * (wave + 1)) / nw);
     const int i16 = lane & 15, k4 = lane >> 4;   // lane = i16 + 16 k4: x / z index, particle slot
     // fixed-point scale of the item's LDS sums: q d 2^sc is folded into the x factor
-    const int sc = __builtin_amdgcn_readfirstlane(flush_scale((double)n * item_qmax_read(wmax, nw) * g.d, g.d));
+    const int sc = __builtin_amdgcn_readfirstlane(flush_scale_i32((double)n * item_qmax_read(wmax, nw) * g.d, g.d));
     const float h = (float)g.h, c = (float)g.c;
     const float dn = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)ldexp(g.d, sc))));
     const float kc = -c * 1.4426950408889634f;
@@ -501,12 +505,12 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
                 // one that indexes the ring registers at run time)
                 asm volatile("; ring slot %0" ::"n"(s));
                 const int y = base + (s >= b ? s - b : s - b + GY);
-                double* p = sub + y * S2 + rofs;
+                int* p = sub + y * S2 + rofs;
                 const v4f v = acc[s];
                 acc[s] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    __hip_atomic_fetch_add(p + r * S, (double)rintf(v[r]), __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_add(p + r * S, (int)rintf(v[r]), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
@@ -580,20 +584,20 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
     float* out = partials + (size_t)item * tl.part_elems;
     SubWalk w(S, tid, nthr);   // canonical [x][y][z]: a = ix, b = iy, c = iz
     for (int e = tid; e < S3; e += nthr) {
-        out[e] = (float)ldexp(sub[(w.b * S + w.a) * S + w.c], -sc);
+        out[e] = ldexpf((float)sub[(w.b * S + w.a) * S + w.c], -sc);
         w.next();
     }
 }
 
 // Column spread of the column items: work-groups of `block` threads (256, 512 or 768; the
-// kernel is held to 168 registers, 3 waves per SIMD), one per CU (the FP64 sub-grid fills
-// the LDS).
+// kernel is held to 168 registers).  At 512 threads two work-groups share a CU (LDS 79 KB
+// each at tile 12, and registers for four waves per SIMD).
 bool launch_spread_col16(const Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items,
                          const Geom& g, const Tiling& tl, float* partials, int dense_min, int block, hipStream_t s)
 {
     SE_REQUIRE(g.P == 16 && tl.S >= 16 + tl.TS - 1 && block % 256 == 0 && block >= 256 && block <= 768, SE_ERR_ARG,
                "column spread: P = 16 and 256, 512 or 768 threads");
-    const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(double);
+    const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(int);
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)k_spread_col16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_spread_col16, dim3(tl.max_items), dim3(block), lds, s, rec, item_beg, item_end, n_items, g,

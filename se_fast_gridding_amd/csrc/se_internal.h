@@ -210,6 +210,16 @@ __device__ __forceinline__ int flush_scale(double bound, double d)
     return s < smax ? s : smax;
 }
 
+// The same for 32-bit integer sums (the column spread's LDS sub-grid): every partial sum
+// stays below 2^30 in magnitude, a quantum of about n max|q| d 2^-29.
+__device__ __forceinline__ int flush_scale_i32(double bound, double d)
+{
+    if (!(bound >= 0x1p-1000) || !(bound < 1e300)) return 0;
+    const int s = 29 - exponent_of(bound);
+    const int smax = 125 - exponent_of(d);
+    return s < smax ? s : smax;
+}
+
 // max |q| over the records [beg, end) of a work item: every thread reduces a strided share,
 // the wave maximum goes to wmax[wave]; after the caller's barrier, item_qmax_read combines
 // them.

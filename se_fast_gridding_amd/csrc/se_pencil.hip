@@ -1158,7 +1158,7 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int
                "spread work-group size past the kernel's bound");
     const int sparse_max = spread_sparse_max(tl);
     const int dense_min = spread_dense_min(tl, g.P);
-    if (dense_min > 0) launch_spread_col16(rec, item_beg, item_end, n_items, g, tl, partials, dense_min, 768, s);
+    if (dense_min > 0) launch_spread_col16(rec, item_beg, item_end, n_items, g, tl, partials, dense_min, 512, s);
     dim3 gr(tl.max_items), bl(tl.sblock);
 #define SE_SP(PV)                                                                                            \
     do {                                                                                                     \
