@@ -260,6 +260,9 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
     if (rb >= re) return;
     v4f rn = *reinterpret_cast<const v4f*>(rec + min(rb + j, re - 1));   // t[0], t[1], t[2], q
     int lon = rec[min(rb + j, re - 1)].lo, orn = rec[min(rb + j, re - 1)].orig;
+    // settled here, so that no load is pending where the loop starts (otherwise the wait the
+    // compiler places at the top of the loop would also cover the previous group's stores)
+    asm volatile("" ::"v"(rn.x), "v"(rn.y), "v"(rn.z), "v"(rn.w), "v"(lon), "v"(orn));
     for (int g0 = rb; g0 < re;) {
         const v4f r = rn;
         const int lo = lon, orig = orn;
@@ -355,6 +358,10 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
             sy = sum_rows4(sy);
             sz = sum_rows4(sz);
         }
+        // Wait for the next group's records here, before this group's output stores: vmcnt
+        // counts loads and stores together, so a wait placed after the stores (at the top of
+        // the next group, where the compiler would put it) would also wait for them.
+        asm volatile("" ::"v"(rn.x), "v"(rn.y), "v"(rn.z), "v"(rn.w), "v"(lon), "v"(orn));
         if (act) {   // one output per lane row
             if (POT && q == 0) store_out(phi, orig, h3 * (double)sp);
             if (FORCE && q > 0) {

@@ -1069,12 +1069,31 @@ __global__ __launch_bounds__(512, 4) void k_interp_row16(const Rec<float>* __res
     const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
     const float kc = -c * 1.4426950408889634f;
     const double h3 = g.h * g.h * g.h;
-    for (int p0 = beg + 4 * wave; p0 < end; p0 += 4 * nw) {   // 4 particles per wave
+    // The next iteration's record fields are loaded one iteration ahead (the output index
+    // early in the iteration) and waited for before this iteration's output stores: vmcnt
+    // counts loads and stores together, so a wait at the top of the loop would also wait for
+    // the previous iteration's stores.
+    typedef float v4f_t __attribute__((ext_vector_type(4)));
+    const int p00 = beg + 4 * wave;
+    v4f_t rn = {0.f, 0.f, 0.f, 0.f};
+    int lon = 0;
+    if (p00 < end) {
+        const int pi = min(p00 + (lane >> 4), end - 1);
+        rn = *reinterpret_cast<const v4f_t*>(rec + pi);
+        lon = rec[pi].lo;
+    }
+    asm volatile("" ::"v"(rn.x), "v"(rn.y), "v"(rn.z), "v"(rn.w), "v"(lon));
+    for (int p0 = p00; p0 < end; p0 += 4 * nw) {   // 4 particles per wave
         const int p = p0 + (lane >> 4);
         const bool valid = p < end;
-        const int pi = valid ? p : p0;
-        const float4 r = *reinterpret_cast<const float4*>(rec + pi);   // t[0], t[1], t[2], q
-        const int lo = rec[pi].lo, orig = rec[pi].orig;
+        const v4f_t r = rn;   // t[0], t[1], t[2], q
+        const int lo = lon;
+        const int orig = rec[valid ? p : p0].orig;
+        {
+            const int pi = min(p + 4 * nw, end - 1);   // the next iteration's records
+            rn = *reinterpret_cast<const v4f_t*>(rec + pi);
+            lon = rec[pi].lo;
+        }
         const int lx = lo & 255, ly = (lo >> 8) & 255, lz = lo >> 16;
         // lane z of the row evaluates window index z of the x and y factors; every lane
         // of the row then takes all 16 from the row's lanes
@@ -1123,6 +1142,7 @@ __global__ __launch_bounds__(512, 4) void k_interp_row16(const Rec<float>* __res
             sy = row_sum(fz * uy);
             szz = row_sum(fz * (-c * az) * u);
         }
+        asm volatile("" ::"v"(rn.x), "v"(rn.y), "v"(rn.z), "v"(rn.w), "v"(lon), "v"(orig));
         if (valid && z == 0) {
             if (POT) store_out(phi, orig, h3 * (double)sp);
             if (FORCE) {
