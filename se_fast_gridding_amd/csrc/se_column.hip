@@ -16,8 +16,8 @@
 // k_column_order (a stable partition of each pencil by column), so that a column's
 // particles are consecutive records in row order: a group is a run of up to sixteen
 // consecutive records of one column, and the work of an item splits into equal record
-// shares per wave.  Dense items are those of dense_min to COLUMN_MAX particles
-// (column_item); the other kernels of the spread and the gather skip exactly these.
+// shares per wave.  k_column_order also decides which items are column items (item_col);
+// the other kernels of the spread and the gather skip exactly these.
 //
 // FP32 matrix instructions and FP32 vector instructions do not overlap on this GPU (one
 // v_mfma_f32_16x16x4f32 and eight v_fma_f32 per iteration take the sum of their times, at
@@ -82,38 +82,77 @@ __device__ __forceinline__ T block_exclusive(T v, T ident, Op op, T* wtot)
 }  // namespace
 
 // ---------------------------------------------------------------------------------
-// Column order of the dense items: each pencil's records, in row order, are partitioned
-// stably by their column (lx mod 4), so the records become (pencil, column, row).  One
-// work-group per item; each thread holds eight consecutive records in registers, the
-// records' window words sit in LDS for the neighbour tests, and record i goes to
+// Column order.  One work-group per item of column_floor to COLUMN_MAX particles; each
+// thread holds eight consecutive records in registers and the records' window words sit in
+// LDS for the neighbour tests.
+//
+// The item is a column item if the column kernels' groups (runs of up to sixteen
+// particles of one 1 x 1 column within five rows) hold COLUMN_GROUP_FILL particles or more
+// on average: their work per group does not shrink with the group, so thinly filled
+// groups are cheaper on the pencil kernels.  For a column item the kernel writes the fixed-point scale of its column spread
+// sums to item_col[item] (COLUMN_NONE for every other item) from a bound on any sub-grid
+// sum, sum_i |q_i| d Fx_i Fy_i Fz_i <= max|q| d B: with n(lx, lz, ly) the particles per
+// window-start cell, gm(w) the largest factor of support point w over a particle's place
+// in its cell, and
+//   Y(lx, lz) = max_y sum_ly n(lx, lz, ly) gm(y - ly),
+//   B = max_{x,z} sum_{lx,lz} Y(lx, lz) gm(x - lx) gm(z - lz)
+// (separable: the z sum first), which for spread-out items is tens of times below the
+// particle count that bounds it crudely, and so buys the 32-bit sums as many bits.
+//
+// The records are then put in column order: each pencil's records, in row order, are
+// partitioned stably by their column (lx mod 4), so the records become (pencil, column,
+// row).  Record i goes to
 //   s + (records of the pencil in lower columns) + (records of its column before it),
 // s the pencil's first record, from per-column running counts (four 16-bit fields of one
 // 64-bit count, scanned over the item) taken at the pencil's start and end.  Every record
 // is read before the first barrier and written after the last, so the permutation is
 // done in place.
 // ---------------------------------------------------------------------------------
+constexpr int COL_RING = 20;     // column spread: grid rows of a wave's register ring
+constexpr int COL_KSTEPS = 4;    // K steps (of four particles) per group
 constexpr int COLUMN_ORDER_THREADS = 256;
 constexpr int COLUMN_PER_THREAD = COLUMN_MAX / COLUMN_ORDER_THREADS;
 
 __device__ __forceinline__ unsigned long long col_one(int lo) { return 1ull << (16 * (lo & 3)); }
 __device__ __forceinline__ int col_field(unsigned long long c, int d) { return (int)((c >> (16 * d)) & 0xFFFF); }
 
+// Largest factor exp(-c a^2) a particle gives support point w over its place in its cell,
+// t in [0, h) (even P) or [-h/2, h/2) (odd P), a = t - (first + w) h: m h is the least
+// |a| over the cell.
+__device__ __forceinline__ float window_factor_max(const Geom& g, int w)
+{
+    const float a = (float)(g.first + w);
+    const float m = g.odd ? fmaxf(fabsf(a) - 0.5f, 0.f) : fmaxf(fmaxf(-a, a - 1.f), 0.f);
+    return expf(-(float)(g.c * g.h * g.h) * m * m);
+}
+
 __global__ __launch_bounds__(COLUMN_ORDER_THREADS) void k_column_order(Rec<float>* __restrict__ rec,
                                                                        const int* __restrict__ item_beg,
                                                                        const int* __restrict__ item_end,
-                                                                       const int* __restrict__ n_items,
-                                                                       int dense_min)
+                                                                       const int* __restrict__ n_items, Geom g,
+                                                                       int TS, int floor_n,
+                                                                       int* __restrict__ item_col)
 {
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], n = item_end[item] - beg;
-    if (!column_item(n, dense_min)) return;
+    if (n < floor_n || n > COLUMN_MAX) {
+        if (threadIdx.x == 0) item_col[item] = COLUMN_NONE;
+        return;
+    }
     __shared__ int los[COLUMN_MAX];                        // the records' window words
     __shared__ unsigned long long cst[COLUMN_MAX];         // per-column counts before a pencil's first record
     __shared__ unsigned long long cen[COLUMN_MAX];         // ... and after its last (indexed by its first)
+    __shared__ int hist[16 * 16 * 16];                     // particles per window-start cell, [lx][lz][ly]
+    __shared__ float ycol[16 * 16];                        // Y(lx, lz)
+    __shared__ float tz[16 * 31];                          // sum_lz Y(lx, lz) gm(z - lz), [lx][z]
+    __shared__ float gmt[32];                              // gm(w), 0 past the support
     __shared__ unsigned int wlo[16], whi[16];
     __shared__ int wmx[16];
-    const int t = threadIdx.x;
+    __shared__ float wq[16], wb[16];
+    __shared__ int ngroups;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int P = g.P, S = TS + P - 1, T3 = TS * TS * TS;
     const int i0 = t * COLUMN_PER_THREAD;   // thread t owns records i0 .. i0 + COLUMN_PER_THREAD - 1
     v4f* rp = reinterpret_cast<v4f*>(rec + beg);   // plain vectors (no HIP vector-type copies)
     v4f ra[COLUMN_PER_THREAD], rb[COLUMN_PER_THREAD];
@@ -122,9 +161,70 @@ __global__ __launch_bounds__(COLUMN_ORDER_THREADS) void k_column_order(Rec<float
         const int i = min(i0 + u, n - 1);   // threads past the item reread its last record
         ra[u] = rp[2 * i];
         rb[u] = rp[2 * i + 1];   // lo, orig, pad
-        if (i0 + u < n) los[i] = __float_as_int(rb[u].x);
+    }
+    for (int e = t; e < T3; e += COLUMN_ORDER_THREADS) hist[e] = 0;
+    if (t < 32) gmt[t] = t < P ? window_factor_max(g, t) : 0.f;
+    if (t == 0) ngroups = 0;
+    __syncthreads();
+    float qm = 0.f;
+#pragma unroll
+    for (int u = 0; u < COLUMN_PER_THREAD; ++u) {
+        const int i = i0 + u;
+        qm = fmaxf(qm, fabsf(ra[u].w));
+        if (i < n) {
+            const int lo = __float_as_int(rb[u].x);
+            los[i] = lo;
+            atomicAdd(&hist[((lo & 255) * TS + (lo >> 16)) * TS + ((lo >> 8) & 255)], 1);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) qm = fmaxf(qm, __shfl_xor(qm, o, 64));
+    if (lane == 0) wq[wave] = qm;
+    __syncthreads();
+
+    // Y of column t (TS <= 16, S <= 31: registers with static indices), and the column
+    // spread's groups in the item
+    if (t < TS * TS) {
+        float hc[16], gw[16];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) {
+            hc[l] = l < TS ? (float)hist[t * TS + l] : 0.f;
+            gw[l] = gmt[l];
+        }
+        float ym = 0.f;
+#pragma unroll
+        for (int y = 0; y < 31; ++y) {
+            float v = 0.f;
+#pragma unroll
+            for (int l = (y > 15 ? y - 15 : 0); l <= (y < 15 ? y : 15); ++l) v += hc[l] * gw[y - l];
+            ym = fmaxf(ym, v);   // rows past the sub-grid add nothing (no terms)
+        }
+        ycol[t] = ym;
+        // the column's groups: runs of up to 16 of its records (in row order) whose rows
+        // lie within COL_RING - 16 of the run's first
+        int* hcol = hist + t * TS;
+        int groups = 0;
+        for (int ly = 0; ly < TS;) {
+            if (hcol[ly] == 0) {
+                ++ly;
+                continue;
+            }
+            int take = 16;
+            for (int r = ly; r < min(ly + COL_RING - 15, TS) && take > 0; ++r) {
+                const int k = min(hcol[r], take);
+                hcol[r] -= k;
+                take -= k;
+            }
+            ++groups;
+        }
+        if (groups) atomicAdd(&ngroups, groups);
     }
     __syncthreads();
+    if ((float)n < COLUMN_GROUP_FILL * (float)ngroups) {   // thinly filled groups: the pencil kernels take the item
+        if (t == 0) item_col[item] = COLUMN_NONE;
+        return;
+    }
+
     // record i starts its pencil (lx / 4, lz)
 #define SE_STARTS(i, lo) ((i) == 0 || ((los[(i) - 1] ^ (lo)) & 0xFF00FC) != 0)
 
@@ -163,7 +263,33 @@ __global__ __launch_bounds__(COLUMN_ORDER_THREADS) void k_column_order(Rec<float
             if (i == n - 1 || ((los[i + 1] ^ lo) & 0xFF00FC) != 0) cen[s] = cv + col_one(lo);
         }
     }
+    for (int e = t; e < TS * S; e += COLUMN_ORDER_THREADS) {
+        const int lx = e / S, z = e - lx * S;
+        float v = 0.f;
+        for (int lz = max(0, z - P + 1); lz <= min(z, TS - 1); ++lz) v += ycol[lx * TS + lz] * gmt[z - lz];
+        tz[e] = v;
+    }
     __syncthreads();
+    float bm = 0.f;
+    for (int e = t; e < S * S; e += COLUMN_ORDER_THREADS) {
+        const int x = e / S, z = e - x * S;
+        float v = 0.f;
+        for (int lx = max(0, x - P + 1); lx <= min(x, TS - 1); ++lx) v += gmt[x - lx] * tz[lx * S + z];
+        bm = fmaxf(bm, v);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) bm = fmaxf(bm, __shfl_xor(bm, o, 64));
+    if (lane == 0) wb[wave] = bm;
+    __syncthreads();
+    if (t == 0) {
+        float b = 0.f, q = 0.f;
+        for (int w = 0; w < COLUMN_ORDER_THREADS / 64; ++w) {
+            b = fmaxf(b, wb[w]);
+            q = fmaxf(q, wq[w]);
+        }
+        item_col[item] = flush_scale_i32((double)q * g.d * b, g.d);
+    }
+
     s = prev_start;
 #pragma unroll
     for (int u = 0; u < COLUMN_PER_THREAD; ++u) {
@@ -186,10 +312,11 @@ __global__ __launch_bounds__(COLUMN_ORDER_THREADS) void k_column_order(Rec<float
 }
 
 void launch_column_order(Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items, int max_items,
-                         int dense_min, hipStream_t s)
+                         const Geom& g, const Tiling& tl, int* item_col, hipStream_t s)
 {
+    SE_REQUIRE(tl.TS <= 16 && g.P <= 16, SE_ERR_ARG, "column order: tile side or support past 16");
     hipLaunchKernelGGL(k_column_order, dim3(max_items), dim3(COLUMN_ORDER_THREADS), 0, s, rec, item_beg, item_end,
-                       n_items, dense_min);
+                       n_items, g, tl.TS, column_floor(tl), item_col);
 }
 
 // ---------------------------------------------------------------------------------
@@ -217,14 +344,15 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
                                                       const int* __restrict__ n_items,
                                                       const float* __restrict__ grid, Geom g, Tiling tl, int N,
                                                       double* __restrict__ phi, double* __restrict__ force,
-                                                      const SysDesc* __restrict__ sys, int dense_min)
+                                                      const SysDesc* __restrict__ sys,
+                                                      const int* __restrict__ item_col)
 {
     constexpr int P = 16;
     const int item = blockIdx.x;
     if (item >= *n_items) return;
-    const int beg = item_beg[item], end = item_end[item], n = end - beg;
     // the other items are the row (or vector) gather's
-    if (!column_item(n, dense_min)) return;
+    if (item_col[item] == COLUMN_NONE) return;
+    const int beg = item_beg[item], end = item_end[item], n = end - beg;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -377,7 +505,8 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
 // Column gather of the column items; block = 256, 512 or 1024.
 void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                          const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
-                         double* phi, double* force, const SysDesc* sys, int dense_min, int block, hipStream_t s)
+                         double* phi, double* force, const SysDesc* sys, const int* item_col, int block,
+                         hipStream_t s)
 {
     SE_REQUIRE(g.P == 16 && tl.S >= 16 + tl.TS - 1 && (block == 256 || block == 512 || block == 1024), SE_ERR_ARG,
                "column gather: P = 16 and 256, 512 or 1024 threads");
@@ -389,7 +518,7 @@ void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int*
             (void)hipFuncSetAttribute((const void*)k_interp_col16<POT, FORCE>,                               \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                 \
         hipLaunchKernelGGL((k_interp_col16<POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg, item_end, \
-                           n_items, grid, g, tl, N, phi, force, sys, dense_min);                             \
+                           n_items, grid, g, tl, N, phi, force, sys, item_col);                              \
     } while (0)
     if (phi && force)
         SE_GC(true, true);
@@ -425,8 +554,6 @@ void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int*
 // (clustered N = 4M at M = 256, dense items: 1.03 against 0.93 ms): the branches cost
 // more than the ring's extra rows.
 // ---------------------------------------------------------------------------------
-constexpr int COL_RING = 20;
-constexpr int COL_KSTEPS = 4;   // K steps (of four particles) per group
 
 // a * (lane L of b's 16-lane row): a DPP row_newbcast:L move (GFX9.4 DPP) into a multiply,
 // which the compiler folds into one v_mul_f32_dpp
@@ -465,29 +592,28 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
                                                       const int* __restrict__ item_beg,
                                                       const int* __restrict__ item_end,
                                                       const int* __restrict__ n_items, Geom g, Tiling tl,
-                                                      float* __restrict__ partials, int dense_min)
+                                                      float* __restrict__ partials,
+                                                      const int* __restrict__ item_col)
 {
     constexpr int P = 16, GY = COL_RING, KS = COL_KSTEPS;
     const int item = blockIdx.x;
     if (item >= *n_items) return;
+    // the other items are the pencil and plane spreads'; a column item's entry is the
+    // fixed-point scale of its sums: q d 2^sc is folded into the x factor
+    const int sc = item_col[item];
+    if (sc == COLUMN_NONE) return;
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
-    // the other items are the pencil and plane spreads'
-    if (!column_item(n, dense_min)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ float wmax[12];
     int* sub = reinterpret_cast<int*>(smem_raw);   // [y][x][z], S^3 fixed-point sums
     const int S = tl.S, S2 = S * S, S3 = S2 * S;
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int e = tid; e < S3; e += nthr) sub[e] = 0;
-    item_qmax_part(rec, beg, end, tid, nthr, wmax);
     __syncthreads();
 
     // equal shares of the item's records, one per wave
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
     const int rb = beg + (int)(((long)n * wave) / nw), re = beg + (int)(((long)n * (wave + 1)) / nw);
     const int i16 = lane & 15, k4 = lane >> 4;   // lane = i16 + 16 k4: x / z index, particle slot
-    // fixed-point scale of the item's LDS sums: q d 2^sc is folded into the x factor
-    const int sc = __builtin_amdgcn_readfirstlane(flush_scale_i32((double)n * item_qmax_read(wmax, nw) * g.d, g.d));
     const float h = (float)g.h, c = (float)g.c;
     const float dn = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)ldexp(g.d, sc))));
     const float kc = -c * 1.4426950408889634f;
@@ -600,7 +726,8 @@ __global__ __launch_bounds__(768) void k_spread_col16(const Rec<float>* __restri
 // kernel is held to 168 registers).  At 512 threads two work-groups share a CU (LDS 79 KB
 // each at tile 12, and registers for four waves per SIMD).
 bool launch_spread_col16(const Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items,
-                         const Geom& g, const Tiling& tl, float* partials, int dense_min, int block, hipStream_t s)
+                         const Geom& g, const Tiling& tl, float* partials, const int* item_col, int block,
+                         hipStream_t s)
 {
     SE_REQUIRE(g.P == 16 && tl.S >= 16 + tl.TS - 1 && block % 256 == 0 && block >= 256 && block <= 768, SE_ERR_ARG,
                "column spread: P = 16 and 256, 512 or 768 threads");
@@ -608,7 +735,7 @@ bool launch_spread_col16(const Rec<float>* rec, const int* item_beg, const int* 
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)k_spread_col16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_spread_col16, dim3(tl.max_items), dim3(block), lds, s, rec, item_beg, item_end, n_items, g,
-                       tl, partials, dense_min);
+                       tl, partials, item_col);
     return true;
 }
 

@@ -163,7 +163,8 @@ struct se_plan {
     // first sorted record
     int *key = nullptr, *idx = nullptr, *key2 = nullptr, *idx2 = nullptr, *tile_start = nullptr, *tile_count = nullptr,
         *nchunk = nullptr, *chunk_start = nullptr, *item_tile = nullptr, *item_beg = nullptr,
-        *item_end = nullptr, *n_items = nullptr, *nseg = nullptr, *nsegz = nullptr;
+        *item_end = nullptr, *n_items = nullptr, *nseg = nullptr, *nsegz = nullptr,
+        *item_col = nullptr;   // column path: per-item column table (COLUMN_NONE or the fixed-point scale)
     int4 *seg = nullptr, *segz = nullptr;
     void *rec = nullptr, *partials = nullptr, *grid = nullptr, *spec = nullptr, *cub_tmp = nullptr,
          *sort_tmp = nullptr, *fft_work = nullptr;
@@ -449,7 +450,7 @@ size_t plan_bytes_estimate(const se_plan_desc* d, int lds_max, int n_sys)
     b += 2 * 256 * 4 * (N / 1024 + 1) + (64u << 10);      // sort histograms and scan area (bound)
     b += (size_t)(ns * tl.ntiles + 1) * 4;                // tile starts
     b += 4 * (size_t)(ns * tl.ntiles + 2) * 4;            // tile_count, nchunk, chunk_start
-    b += 3 * (size_t)tl.max_items * 4;                    // items
+    b += 4 * (size_t)tl.max_items * 4;                    // items (tile, begin, end, column table)
     b += N * (es == 4 ? 32 : 48);                         // records
     b += (size_t)tl.max_items * tl.part_elems * es;       // partials
     b += ns * M * M * Mz * es;                            // grids
@@ -504,18 +505,20 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int
     // work items: at most one partial item per non-empty tile plus one per full chunk
     Tiling tlc = tl;
     tlc.max_items = std::min(tl.max_items, nt + N / tl.chunk + 1);
-    // the column items' records into column order, for the column spread and gather
+    // the column items: chosen, and their records put in column order, for the column
+    // spread and gather
     if constexpr (sizeof(T) == 4) {
         if (tl.column)
-            launch_column_order((Rec<float>*)pl->rec, pl->item_beg, pl->item_end, pl->n_items, tlc.max_items,
-                                column_dense_min(tl), s);
+            launch_column_order((Rec<float>*)pl->rec, pl->item_beg, pl->item_end, pl->n_items, tlc.max_items, pl->g,
+                                tl, pl->item_col, s);
     }
     pl->rec_ev(1);
     bool done = false;
     if constexpr (sizeof(T) == 4) {
         if (tl.pencil)
             done = launch_spread_pencil((const Rec<float>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end,
-                                        pl->n_items, pl->tile_start, pl->g, tlc, (float*)pl->partials, s);
+                                        pl->n_items, pl->tile_start, pl->g, tlc, (float*)pl->partials,
+                                        tl.column ? pl->item_col : nullptr, s);
     }
     if (!done)
         launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
@@ -552,7 +555,8 @@ void run_interp(se_plan* pl, double* phi, double* force)
         if (pl->tl.pencil)
             done = launch_interp_pencil((const Rec<float>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end,
                                         pl->n_items, (const float*)pl->grid, pl->g, tlc, pl->last_N, phi, force, sys,
-                                        pl->stream, split ? pl->side : nullptr, pl->fork_ev, pl->join_ev);
+                                        tl.column ? pl->item_col : nullptr, pl->stream, split ? pl->side : nullptr,
+                                        pl->fork_ev, pl->join_ev);
     }
     if (!done)
         launch_interp_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,
@@ -745,6 +749,7 @@ static int create_plan(se_plan** out, const se_plan_desc* d, void* stream, int n
         pl->item_tile = (int*)pl->alloc((size_t)tl.max_items * 4);
         pl->item_beg = (int*)pl->alloc((size_t)tl.max_items * 4);
         pl->item_end = (int*)pl->alloc((size_t)tl.max_items * 4);
+        pl->item_col = (int*)pl->alloc((size_t)tl.max_items * 4);
         pl->n_items = (int*)pl->alloc(16);
         pl->rec = pl->alloc(N * (pl->esize == 4 ? sizeof(Rec<float>) : sizeof(Rec<double>)));
         pl->partials = pl->alloc((size_t)tl.max_items * tl.part_elems * pl->esize);

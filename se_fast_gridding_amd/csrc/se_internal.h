@@ -249,8 +249,8 @@ constexpr int spread_pencil_max_threads(int P) { return P == 16 ? 768 : 1024; }
 // the vector gather takes those items instead of the matrix-core gather (a 4-particle
 // MFMA group needs about one particle per pencil row).
 inline int sparse_tile_max(const Tiling& tl) { return tl.K / 2 > 1 ? tl.K / 2 : 1; }
-// P = 16 FP32 gather split: items with fewer particles than this take the row gather,
-// the others the matrix-core gather (se_pencil.hip)
+// P = 16 FP32 gather split without the column path: items with fewer particles than this
+// take the row gather, the others the matrix-core gather (se_pencil.hip)
 int gather_dense_min(const Tiling& tl);
 
 // Particle record written by the bucketing scatter, read by spread and gather.
@@ -343,9 +343,10 @@ void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, 
 bool pencil_supported(int P);
 // pencil spread; where tl.plane is set, whole sparse tiles take the plane spread
 // (tile_start: the tiles' first records)
+// item_col: the column path's per-item table (launch_column_order), or nullptr
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const int* tile_start, const Geom& g, const Tiling& tl,
-                          float* partials, hipStream_t s);
+                          float* partials, const int* item_col, hipStream_t s);
 // sys != null: grouped solve, the outputs and N come from sys[tile / ntiles] and the
 // grids are stacked; phi/force then only select the output mode.  side (with the events
 // fork and join) non-null: where the items split between two kernels, the sparse items'
@@ -353,31 +354,43 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int
 bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg,
                           const int* item_end, const int* n_items, const float* grid,
                           const Geom& g, const Tiling& tl, int N, double* phi, double* force,
-                          const SysDesc* sys, hipStream_t s, hipStream_t side = nullptr,
+                          const SysDesc* sys, const int* item_col, hipStream_t s, hipStream_t side = nullptr,
                           hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
-// Column items (the dense work items of the P = 16 FP32 path with matrix cores): dense_min
-// to COLUMN_MAX particles.  k_column_order puts their records in column order (staged in
-// LDS, so their number is capped); the column spread and gather take them and every other
-// spread and gather kernel skips them (dense_min 0: no column items).
-constexpr int COLUMN_MAX = 2048;
-__host__ __device__ inline bool column_item(int n, int dense_min)
-{
-    return dense_min > 0 && n >= dense_min && n <= COLUMN_MAX;
-}
-// the column items' threshold: 2.5 particles per (pencil, row) key
-inline int column_dense_min(const Tiling& tl) { return 5 * tl.K / 2; }
-// records of the column items into column order (pencil, lx mod 4, row), in place
+// Dense items: those the matrix-core spread and gather take; every other spread and gather
+// kernel skips them.  On the column path (P = 16 FP32, tl.column) k_column_order decides per
+// item and writes item_col[item]: COLUMN_NONE, or the fixed-point scale of the item's
+// column spread sums; the other path with matrix cores (the MFMA gather) goes by size.
+constexpr int COLUMN_MAX = 2048;                  // column items are staged in LDS: at most this many records
+constexpr int COLUMN_NONE = -2147483647 - 1;      // item_col entry of an item the column kernels leave
+constexpr float COLUMN_GROUP_FILL = 5.5f;         // column items: at least this many particles per column group
+struct DenseSel {
+    const int* col;   // per-item column table (column path) or nullptr
+    int min;          // without a table: dense from min to COLUMN_MAX particles (0: none)
+    __device__ __forceinline__ bool operator()(int item, int n) const
+    {
+        if (col) return col[item] != COLUMN_NONE;
+        return min > 0 && n >= min && n <= COLUMN_MAX;
+    }
+};
+// column path: the items considered for column order (from one particle per (pencil, row)
+// key to COLUMN_MAX particles)
+inline int column_floor(const Tiling& tl) { return tl.K; }
+// Column order of the items from column_floor to COLUMN_MAX particles whose column groups
+// hold COLUMN_GROUP_FILL particles on average: records into (pencil, lx mod 4, row)
+// order, in place, and item_col[item] for every item (COLUMN_NONE for the others)
 void launch_column_order(Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items, int max_items,
-                         int dense_min, hipStream_t s);
-// column gather (se_column.hip): column items, P = 16 FP32, work-groups of 256, 512 or
-// 1024 threads
+                         const Geom& g, const Tiling& tl, int* item_col, hipStream_t s);
+// column gather (se_column.hip): the items of item_col, P = 16 FP32, work-groups of 256,
+// 512 or 1024 threads
 void launch_interp_col16(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                          const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
-                         double* phi, double* force, const SysDesc* sys, int dense_min, int block, hipStream_t s);
-// column spread (se_column.hip): column items, P = 16 FP32, work-groups of 256, 512 or
-// 768 threads
+                         double* phi, double* force, const SysDesc* sys, const int* item_col, int block,
+                         hipStream_t s);
+// column spread (se_column.hip): the items of item_col, P = 16 FP32, work-groups of 256,
+// 512 or 768 threads
 bool launch_spread_col16(const Rec<float>* rec, const int* item_beg, const int* item_end, const int* n_items,
-                         const Geom& g, const Tiling& tl, float* partials, int dense_min, int block, hipStream_t s);
+                         const Geom& g, const Tiling& tl, float* partials, const int* item_col, int block,
+                         hipStream_t s);
 template <typename T>
 void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* item_beg,
                          const int* item_end, const int* n_items, const Geom& g,

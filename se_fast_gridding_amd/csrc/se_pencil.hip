@@ -265,7 +265,7 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
                                                        const int* __restrict__ n_items, Geom g, Tiling tl,
                                                        float* __restrict__ partials, const int* __restrict__ item_tile,
                                                        const int* __restrict__ tile_start, int sparse_max,
-                                                       int dense_min)
+                                                       DenseSel dense)
 {
     constexpr int R = ((P + 3) * P + 63) / 64;
     const int item = blockIdx.x;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(spread_pencil_max_threads(P)) void k_spread_pencil(
     // sparse items (whole tiles of fewer than sparse_max particles) are the plane
     // spread's (sparse_max > 0), the column items the column spread's
     if (sparse_item(item_tile, item_beg, item_end, tile_start, item, sparse_max)) return;
-    if (column_item(item_end[item] - item_beg[item], dense_min)) return;
+    if (dense(item, item_end[item] - item_beg[item])) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ float wmax[spread_pencil_max_threads(P) / 64];
     double* sub = reinterpret_cast<double*>(smem_raw);   // [y][x][z], S^3
@@ -549,14 +549,14 @@ __global__ __launch_bounds__(512) void k_interp_pencil(const Rec<float>* __restr
                                                        const int* __restrict__ n_items,
                                                        const float* __restrict__ grid, Geom g, Tiling tl, int N,
                                                        double* __restrict__ phi, double* __restrict__ force,
-                                                       const SysDesc* __restrict__ sys, int dense_min)
+                                                       const SysDesc* __restrict__ sys, DenseSel dense)
 {
     constexpr int R = ((P + 3) * P + 63) / 64;
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
     // the column items are the matrix-core gather's
-    if (column_item(n, dense_min)) return;
+    if (dense(item, n)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -796,14 +796,14 @@ __global__ __launch_bounds__(512) void k_interp_mfma16(const Rec<float>* __restr
                                                        const int* __restrict__ n_items,
                                                        const float* __restrict__ grid, Geom g, Tiling tl, int N,
                                                        double* __restrict__ phi, double* __restrict__ force,
-                                                       const SysDesc* __restrict__ sys, int dense_min)
+                                                       const SysDesc* __restrict__ sys, DenseSel dense)
 {
     constexpr int P = 16, GY = 20, R = 5;
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
     // the column items only; the others are the row or vector gather's
-    if (!column_item(n, dense_min)) return;
+    if (!dense(item, n)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], S^3
     const int S = tl.S, S2 = S * S, S3 = S2 * S, M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -1038,14 +1038,14 @@ __global__ __launch_bounds__(512, 4) void k_interp_row16(const Rec<float>* __res
                                                          const int* __restrict__ n_items,
                                                          const float* __restrict__ grid, Geom g, Tiling tl, int N,
                                                          double* __restrict__ phi, double* __restrict__ force,
-                                                         const SysDesc* __restrict__ sys, int dense_min)
+                                                         const SysDesc* __restrict__ sys, DenseSel dense)
 {
     constexpr int P = 16, S2 = SS * SS;
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], end = item_end[item], n = end - beg;
     // the column items are the matrix-core gather's
-    if (column_item(n, dense_min)) return;
+    if (dense(item, n)) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float* sub = reinterpret_cast<float*>(smem_raw);   // [y][x][z], SS^3
     const int M = g.M, Mz = g.Mz, NT = tl.NT, NTz = tl.NTz;
@@ -1166,25 +1166,24 @@ static void allow_lds_p(const void* fn, size_t bytes)
 // the pencil spread; 0 turns the plane spread off.
 int spread_sparse_max(const Tiling& tl) { return tl.plane ? sparse_tile_max(tl) : 0; }
 
-// Threshold of the column items the column spread takes (0: none).
-static int spread_dense_min(const Tiling& tl, int P) { return (tl.column && P == 16) ? column_dense_min(tl) : 0; }
 
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const int* tile_start, const Geom& g, const Tiling& tl, float* partials,
-                          hipStream_t s)
+                          const int* item_col, hipStream_t s)
 {
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(double);
     SE_REQUIRE(tl.sblock >= 64 && tl.sblock % 64 == 0 && tl.sblock <= spread_pencil_max_threads(g.P), SE_ERR_ARG,
                "spread work-group size past the kernel's bound");
     const int sparse_max = spread_sparse_max(tl);
-    const int dense_min = spread_dense_min(tl, g.P);
-    if (dense_min > 0) launch_spread_col16(rec, item_beg, item_end, n_items, g, tl, partials, dense_min, 512, s);
+    // the column items (P = 16 with the column path) take the column spread
+    const DenseSel dense{(tl.column && g.P == 16) ? item_col : nullptr, 0};
+    if (dense.col) launch_spread_col16(rec, item_beg, item_end, n_items, g, tl, partials, dense.col, 512, s);
     dim3 gr(tl.max_items), bl(tl.sblock);
 #define SE_SP(PV)                                                                                            \
     do {                                                                                                     \
         allow_lds_p((const void*)k_spread_pencil<PV>, lds);                                                  \
         hipLaunchKernelGGL(k_spread_pencil<PV>, gr, bl, lds, s, rec, item_beg, item_end, n_items, g, tl,    \
-                           partials, item_tile, tile_start, sparse_max, dense_min);                           \
+                           partials, item_tile, tile_start, sparse_max, dense);                               \
         if (sparse_max > 0)                                                                                  \
             hipLaunchKernelGGL(k_spread_plane<PV>, gr, dim3(64 * ((tl.S + 1) / 2)), 0, s, rec, item_tile,      \
                                item_beg, item_end, n_items, tile_start, g, tl, partials, sparse_max);         \
@@ -1201,7 +1200,7 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int
 static void interp_mfma16_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg,
                                 const int* item_end, const int* n_items, const float* grid, const Geom& g,
                                 const Tiling& tl, int N, double* phi, double* force, const SysDesc* sys,
-                                int dense_min, hipStream_t s)
+                                DenseSel dense, hipStream_t s)
 {
     const size_t lds = ((size_t)tl.S * tl.S * tl.S + LDS_TILE_PAD) * sizeof(float);
     dim3 gr(tl.max_items), bl(tl.mblock);
@@ -1209,7 +1208,7 @@ static void interp_mfma16_modes(const Rec<float>* rec, const int* item_tile, con
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_mfma16<POT, FORCE>, lds);                                           \
         hipLaunchKernelGGL((k_interp_mfma16<POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg, item_end, \
-                           n_items, grid, g, tl, N, phi, force, sys, dense_min);                              \
+                           n_items, grid, g, tl, N, phi, force, sys, dense);                                  \
     } while (0)
     if (phi && force)
         SE_GM(true, true);
@@ -1224,7 +1223,7 @@ template <int PV>
 static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg,
                                 const int* item_end, const int* n_items, const float* grid, const Geom& g,
                                 const Tiling& tl, int N, double* phi, double* force, const SysDesc* sys,
-                                int dense_min, hipStream_t s)
+                                DenseSel dense, hipStream_t s)
 {
     const size_t lds = ((size_t)tl.S * tl.S * tl.S + LDS_TILE_PAD) * sizeof(float);
     dim3 gr(tl.max_items), bl(tl.block);
@@ -1232,7 +1231,7 @@ static void interp_pencil_modes(const Rec<float>* rec, const int* item_tile, con
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_pencil<PV, POT, FORCE>, lds);                                       \
         hipLaunchKernelGGL((k_interp_pencil<PV, POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg,       \
-                           item_end, n_items, grid, g, tl, N, phi, force, sys, dense_min);                    \
+                           item_end, n_items, grid, g, tl, N, phi, force, sys, dense);                        \
     } while (0)
     if (phi && force)
         SE_GP(true, true);
@@ -1255,7 +1254,6 @@ static bool row16_supported(const Tiling& tl) { return tl.S == 19 || tl.S == 23 
 int gather_dense_min(const Tiling& tl)
 {
     if (!tl.mfma) return 0;
-    if (tl.column) return column_dense_min(tl);   // the items put in column order
     return row16_supported(tl) ? 5 * tl.K / 2 : sparse_tile_max(tl);
 }
 
@@ -1263,7 +1261,7 @@ int gather_dense_min(const Tiling& tl)
 // 4, 8 and 12); false if the sub-grid side has no instance.
 static bool interp_row16_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                               const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
-                              double* phi, double* force, const SysDesc* sys, int dense_min, hipStream_t s)
+                              double* phi, double* force, const SysDesc* sys, DenseSel dense, hipStream_t s)
 {
     const size_t lds = ((size_t)tl.S * tl.S * tl.S + LDS_TILE_PAD) * sizeof(float);
     dim3 gr(tl.max_items), bl(512);
@@ -1271,7 +1269,7 @@ static bool interp_row16_modes(const Rec<float>* rec, const int* item_tile, cons
     do {                                                                                                      \
         allow_lds_p((const void*)k_interp_row16<SV, POT, FORCE>, lds);                                         \
         hipLaunchKernelGGL((k_interp_row16<SV, POT, FORCE>), gr, bl, lds, s, rec, item_tile, item_beg,         \
-                           item_end, n_items, grid, g, tl, N, phi, force, sys, dense_min);                    \
+                           item_end, n_items, grid, g, tl, N, phi, force, sys, dense);                        \
     } while (0)
 #define SE_GQS(SV)                        \
     do {                                  \
@@ -1294,22 +1292,23 @@ static bool interp_row16_modes(const Rec<float>* rec, const int* item_tile, cons
 
 bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const float* grid, const Geom& g, const Tiling& tl, int N,
-                          double* phi, double* force, const SysDesc* sys, hipStream_t s, hipStream_t side,
-                          hipEvent_t fork, hipEvent_t join)
+                          double* phi, double* force, const SysDesc* sys, const int* item_col, hipStream_t s,
+                          hipStream_t side, hipEvent_t fork, hipEvent_t join)
 {
 #define SE_GPM(PV, DM) \
     interp_pencil_modes<PV>(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys, DM, s)
     switch (g.P) {
-    case 8: SE_GPM(8, 0); return true;
-    case 11: SE_GPM(11, 0); return true;
+    case 8: SE_GPM(8, (DenseSel{nullptr, 0})); return true;
+    case 11: SE_GPM(11, (DenseSel{nullptr, 0})); return true;
     case 16:
         if (tl.mfma) {
-            // Items below gather_dense_min take the row gather (or, where it has no
-            // instance, the vector gather, which also runs two work-groups of at most 256
-            // threads per CU, one loading its sub-grid while the other computes); the
-            // others take the MFMA gather.  The two kernels write disjoint particles; given
+            // The dense items -- the column items with the column path, else those from
+            // gather_dense_min particles -- take the matrix-core gather; the others take
+            // the row gather (or, where it has no instance, the vector gather, which also
+            // runs two work-groups of at most 256 threads per CU, one loading its sub-grid
+            // while the other computes).  The two kernels write disjoint particles; given
             // a side stream they run concurrently, so each fills the other's tail.
-            const int dense_min = gather_dense_min(tl);
+            const DenseSel dense{tl.column ? item_col : nullptr, tl.column ? 0 : gather_dense_min(tl)};
             Tiling tv = tl;
             tv.block = tl.mblock;
             hipStream_t sv = s;
@@ -1319,21 +1318,21 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
                 sv = side;
             }
             if (!interp_row16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
-                                   dense_min, sv))
+                                   dense, sv))
                 interp_pencil_modes<16>(rec, item_tile, item_beg, item_end, n_items, grid, g, tv, N, phi, force, sys,
-                                        dense_min, sv);
+                                        dense, sv);
             if (side) SE_HIP(hipEventRecord(join, side));
             // the column gather at 512 threads: 4 waves per SIMD (two work-groups per CU by
             // LDS); clustered N = 4M at M = 256: 256 threads 1.17 ms, 512 0.99, 1024 1.08
             if (tl.column)
                 launch_interp_col16(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
-                                    dense_min, 512, s);
+                                    item_col, 512, s);
             else
                 interp_mfma16_modes(rec, item_tile, item_beg, item_end, n_items, grid, g, tl, N, phi, force, sys,
-                                    dense_min, s);
+                                    dense, s);
             if (side) SE_HIP(hipStreamWaitEvent(s, join, 0));
         } else {
-            SE_GPM(16, 0);
+            SE_GPM(16, (DenseSel{nullptr, 0}));
         }
         return true;
     default: return false;

@@ -263,9 +263,12 @@ def test_column_kernels_match_pencil_kernels(oracle, clustered):
     """The P = 16 column kernels (sixteen particles of one 1 x 1 window-start column per
     matrix-core product) against the 4 x 1 pencil kernels (SE_PLAN_NO_COLUMN) and the FP64
     oracle, for each output mode, with items that cut columns mid-way (chunk 45 and 301,
-    tile sides 12, 8 and 4, dense thresholds 1080, 360 and 40 particles), work-groups of
-    64 to 512 threads; and bit for bit on a repeat."""
-    N, M, P, xi = 6000, 32, 16, 8.0
+    tile sides 12, 8 and 4), work-groups of 64 to 512 threads; and bit for bit on a repeat.
+    At 1.8 particles per grid cell the column groups are full enough for the column path
+    to take items at every tiling but chunk 45 (below the column floor): its 32-bit
+    fixed-point spread sums make the results differ from the pencil path's in the last
+    bits, which shows that it ran."""
+    N, M, P, xi = 60000, 32, 16, 8.0
     x, q = system(N, 320 + int(clustered), clustered=clustered)
     m = 0.95 * np.sqrt(np.pi * P)
     ref_phi = oracle.se3p(x, q, 1.0, M, P, m, xi)
@@ -292,6 +295,7 @@ def test_column_kernels_match_pencil_kernels(oracle, clustered):
             out[col] = res
         a, b = out[True], out[False]
         print("column vs pencil", kw, [rms_rel(u, v) for u, v in zip(a, b)], flush=True)
+        assert np.array_equal(a[0], b[0]) == (kw.get("chunk") == 45), kw   # the column path ran (or could not)
         for u, v in zip(a, b):
             assert rms_rel(u, v) < 2e-6, kw
         print("potential-only, force-only vs joint: equal %s %s" % (np.array_equal(a[2], a[0]),
