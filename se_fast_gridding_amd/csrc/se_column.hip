@@ -136,7 +136,17 @@ __global__ __launch_bounds__(COLUMN_ORDER_THREADS) void k_column_order(Rec<float
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     const int beg = item_beg[item], n = item_end[item] - beg;
-    if (n < floor_n || n > COLUMN_MAX) {
+    // The records are in (pencil, row) key order, so the item spans keys key(first) to
+    // key(last); spread evenly over them its particles would fill a column group (five rows
+    // of one of a pencil's four columns) with 1.25 n / span.  Items below half the fill
+    // threshold on that estimate are left without loading their records.
+    bool take = n >= floor_n && n <= COLUMN_MAX;
+    if (take) {
+        const auto key = [TS](int lo) { return (((lo & 255) >> 2) * TS + (lo >> 16)) * TS + ((lo >> 8) & 255); };
+        const int span = key(rec[beg + n - 1].lo) - key(rec[beg].lo) + 1;
+        take = (float)n >= 0.4f * COLUMN_GROUP_FILL * (float)span;
+    }
+    if (!take) {
         if (threadIdx.x == 0) item_col[item] = COLUMN_NONE;
         return;
     }
