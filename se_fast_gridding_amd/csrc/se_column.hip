@@ -195,10 +195,11 @@ __global__ __launch_bounds__(COLUMN_ORDER_THREADS) void k_column_order(Rec<float
     // Y of column t (TS <= 16, S <= 31: registers with static indices), and the column
     // spread's groups in the item
     if (t < TS * TS) {
-        float hc[16], gw[16];
+        int hc[16];
+        float gw[16];
 #pragma unroll
         for (int l = 0; l < 16; ++l) {
-            hc[l] = l < TS ? (float)hist[t * TS + l] : 0.f;
+            hc[l] = l < TS ? hist[t * TS + l] : 0;
             gw[l] = gmt[l];
         }
         float ym = 0.f;
@@ -206,26 +207,26 @@ __global__ __launch_bounds__(COLUMN_ORDER_THREADS) void k_column_order(Rec<float
         for (int y = 0; y < 31; ++y) {
             float v = 0.f;
 #pragma unroll
-            for (int l = (y > 15 ? y - 15 : 0); l <= (y < 15 ? y : 15); ++l) v += hc[l] * gw[y - l];
+            for (int l = (y > 15 ? y - 15 : 0); l <= (y < 15 ? y : 15); ++l) v += (float)hc[l] * gw[y - l];
             ym = fmaxf(ym, v);   // rows past the sub-grid add nothing (no terms)
         }
         ycol[t] = ym;
         // the column's groups: runs of up to 16 of its records (in row order) whose rows
-        // lie within COL_RING - 16 of the run's first
-        int* hcol = hist + t * TS;
-        int groups = 0;
-        for (int ly = 0; ly < TS;) {
-            if (hcol[ly] == 0) {
-                ++ly;
-                continue;
+        // lie within COL_RING - 16 of the run's first, counted row by row (static indices)
+        int groups = 0, gs = -COL_RING, cap = 0;   // the open group's first row and room
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            int c = hc[r];
+            while (c > 0) {
+                if (cap == 0 || r > gs + (COL_RING - 16)) {
+                    ++groups;
+                    gs = r;
+                    cap = 16;
+                }
+                const int k = min(c, cap);
+                c -= k;
+                cap -= k;
             }
-            int take = 16;
-            for (int r = ly; r < min(ly + COL_RING - 15, TS) && take > 0; ++r) {
-                const int k = min(hcol[r], take);
-                hcol[r] -= k;
-                take -= k;
-            }
-            ++groups;
         }
         if (groups) atomicAdd(&ngroups, groups);
     }

@@ -518,7 +518,9 @@ void run_spread(se_plan* pl, int N, const double* x, const double* q, int S, int
         if (tl.pencil)
             done = launch_spread_pencil((const Rec<float>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end,
                                         pl->n_items, pl->tile_start, pl->g, tlc, (float*)pl->partials,
-                                        tl.column ? pl->item_col : nullptr, s);
+                                        tl.column ? pl->item_col : nullptr, s,
+                                        (tl.column && !grp && N >= (1 << 20)) ? pl->side : nullptr, pl->fork_ev,
+                                        pl->join_ev);
     }
     if (!done)
         launch_spread_tiles<T>((const Rec<T>*)pl->rec, pl->item_tile, pl->item_beg, pl->item_end, pl->n_items,

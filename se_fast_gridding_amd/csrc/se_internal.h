@@ -343,10 +343,13 @@ void launch_scatter_group(const SysDesc* sys, int S, int blocks, const Geom& g, 
 bool pencil_supported(int P);
 // pencil spread; where tl.plane is set, whole sparse tiles take the plane spread
 // (tile_start: the tiles' first records)
-// item_col: the column path's per-item table (launch_column_order), or nullptr
+// item_col: the column path's per-item table (launch_column_order), or nullptr.  side
+// (with the events fork and join) non-null: the column spread runs on s and the other
+// items' spreads on side, concurrently, and s waits for them.
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const int* tile_start, const Geom& g, const Tiling& tl,
-                          float* partials, const int* item_col, hipStream_t s);
+                          float* partials, const int* item_col, hipStream_t s, hipStream_t side = nullptr,
+                          hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 // sys != null: grouped solve, the outputs and N come from sys[tile / ntiles] and the
 // grids are stacked; phi/force then only select the output mode.  side (with the events
 // fork and join) non-null: where the items split between two kernels, the sparse items'

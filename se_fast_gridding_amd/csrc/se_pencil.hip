@@ -1169,32 +1169,47 @@ int spread_sparse_max(const Tiling& tl) { return tl.plane ? sparse_tile_max(tl) 
 
 bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int* item_beg, const int* item_end,
                           const int* n_items, const int* tile_start, const Geom& g, const Tiling& tl, float* partials,
-                          const int* item_col, hipStream_t s)
+                          const int* item_col, hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join)
 {
+    if (!pencil_supported(g.P)) return false;
     const size_t lds = (size_t)tl.S * tl.S * tl.S * sizeof(double);
     SE_REQUIRE(tl.sblock >= 64 && tl.sblock % 64 == 0 && tl.sblock <= spread_pencil_max_threads(g.P), SE_ERR_ARG,
                "spread work-group size past the kernel's bound");
     const int sparse_max = spread_sparse_max(tl);
-    // the column items (P = 16 with the column path) take the column spread
+    // The column items (P = 16 with the column path) take the column spread; given a side
+    // stream, the pencil and plane spreads of the other items run on it concurrently (all
+    // three write disjoint partials), and s waits for them.
     const DenseSel dense{(tl.column && g.P == 16) ? item_col : nullptr, 0};
-    if (dense.col) launch_spread_col16(rec, item_beg, item_end, n_items, g, tl, partials, dense.col, 512, s);
+    hipStream_t sv = s;
+    if (dense.col) {
+        if (side) {
+            SE_HIP(hipEventRecord(fork, s));
+            SE_HIP(hipStreamWaitEvent(side, fork, 0));
+            sv = side;
+        }
+        launch_spread_col16(rec, item_beg, item_end, n_items, g, tl, partials, dense.col, 512, s);
+    }
     dim3 gr(tl.max_items), bl(tl.sblock);
 #define SE_SP(PV)                                                                                            \
     do {                                                                                                     \
         allow_lds_p((const void*)k_spread_pencil<PV>, lds);                                                  \
-        hipLaunchKernelGGL(k_spread_pencil<PV>, gr, bl, lds, s, rec, item_beg, item_end, n_items, g, tl,    \
+        hipLaunchKernelGGL(k_spread_pencil<PV>, gr, bl, lds, sv, rec, item_beg, item_end, n_items, g, tl,   \
                            partials, item_tile, tile_start, sparse_max, dense);                               \
         if (sparse_max > 0)                                                                                  \
-            hipLaunchKernelGGL(k_spread_plane<PV>, gr, dim3(64 * ((tl.S + 1) / 2)), 0, s, rec, item_tile,      \
+            hipLaunchKernelGGL(k_spread_plane<PV>, gr, dim3(64 * ((tl.S + 1) / 2)), 0, sv, rec, item_tile,     \
                                item_beg, item_end, n_items, tile_start, g, tl, partials, sparse_max);         \
     } while (0)
     switch (g.P) {
-    case 8: SE_SP(8); return true;
-    case 11: SE_SP(11); return true;
-    case 16: SE_SP(16); return true;
-    default: return false;
+    case 8: SE_SP(8); break;
+    case 11: SE_SP(11); break;
+    default: SE_SP(16); break;
     }
 #undef SE_SP
+    if (sv != s) {
+        SE_HIP(hipEventRecord(join, sv));
+        SE_HIP(hipStreamWaitEvent(s, join, 0));
+    }
+    return true;
 }
 
 static void interp_mfma16_modes(const Rec<float>* rec, const int* item_tile, const int* item_beg,
