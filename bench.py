@@ -302,9 +302,11 @@ def roofline(r, workload, prec_name):
                          "frac": fl / (kms * 1e-3) / 1e12 / peak, "counters": meas or None}
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     return {"bound": "mfma",
-            "compute": ("FP32: matrix cores v_mfma_f32_16x16x4f32 (column spread and gather of the dense items) and "
-                        "vector FMA (pencil, plane and row kernels of the others); peak is MI355X dense FP32, "
-                        "157.3 TFLOP/s, the same for both units (on this GPU they do not run at once)"),
+            "compute": ("FP32 results: matrix cores for the dense items (column spread: v_mfma_f32_16x16x4f32; "
+                        "column gather: its z contraction as three v_mfma_f32_16x16x16_f16 on fp16 (hi, lo) pairs "
+                        "of the FP32 operands, hi*hi + lo*hi + hi*lo with FP32 sums, about 2^-22 per product) and "
+                        "vector FP32 FMA (pencil, plane and row kernels of the others); peak is MI355X dense FP32, "
+                        "157.3 TFLOP/s (the FP32 rate of both units), so frac is against FP32 arithmetic"),
             "achieved": flops / t_meas / 1e12, "peak": peak, "unit": "TFLOP/s",
             "frac": t_roof / t_meas, "t_roof_ms": t_roof * 1e3, "t_measured_ms": t_meas * 1e3,
             "flops": flops, "bytes_algorithmic": bytes_,

@@ -32,6 +32,8 @@ namespace se {
 namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef _Float16 v4h __attribute__((ext_vector_type(4)));
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
 
 // exp(-c a^2) as v_exp_f32 of kc a^2, kc = -c log2(e)
 __device__ __forceinline__ float gauss_c(float kc, float a) { return __builtin_amdgcn_exp2f(kc * a * a); }
@@ -50,6 +52,32 @@ __device__ __forceinline__ float sum_rows4(float v)
 __device__ __forceinline__ v4f mfma16(float a, float b, v4f c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// D += A B on 16 x 16 x 16 fp16 operands (FP32 sums)
+__device__ __forceinline__ v4f mfma16h(v4h a, v4h b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// fp16 pair of v: hi = fp16(v), lo = fp16(v - hi), so that hi + lo holds v to about 2^-22
+// of |v| (or 2^-25 absolute where lo is subnormal)
+__device__ __forceinline__ void split_f16(float v, _Float16& hi, _Float16& lo)
+{
+    hi = (_Float16)v;
+    lo = (_Float16)(v - (float)hi);
+}
+
+// four (hi | lo << 16) words -> the four his and the four los as fp16 vectors
+__device__ __forceinline__ v4h pack_hi(unsigned p0, unsigned p1, unsigned p2, unsigned p3)
+{
+    const v2u u = {__builtin_amdgcn_perm(p1, p0, 0x05040100u), __builtin_amdgcn_perm(p3, p2, 0x05040100u)};
+    return __builtin_bit_cast(v4h, u);
+}
+__device__ __forceinline__ v4h pack_lo(unsigned p0, unsigned p1, unsigned p2, unsigned p3)
+{
+    const v2u u = {__builtin_amdgcn_perm(p1, p0, 0x07060302u), __builtin_amdgcn_perm(p3, p2, 0x07060302u)};
+    return __builtin_bit_cast(v4h, u);
 }
 
 __device__ __forceinline__ float bperm_f(int byte_addr, float v)
@@ -382,15 +410,42 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
         load_tile_dma<true>(sub, grid, S, M, Mz, ox, oy, oz, tid, nthr);
     }
     __syncthreads();
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
+
+    // The sub-grid becomes fp16 pairs in place: G 2^sg = hi + lo, one (hi | lo << 16) word
+    // per point, sg such that max |G| 2^sg lies in [2^14, 2^15).
+    __shared__ float wmax[16];
+    const int S3 = S2 * S;
+    {
+        float m = 0.f;
+        for (int e = tid; e < S3; e += nthr) m = fmaxf(m, fabsf(sub[e]));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if (lane == 0) wmax[wave] = m;
+    }
+    __syncthreads();
+    float gmax = 0.f;
+    for (int w = 0; w < nw; ++w) gmax = fmaxf(gmax, wmax[w]);
+    const int sg = gmax > 0.f ? 14 - ilogbf(gmax) : 0;
+    unsigned* subp = reinterpret_cast<unsigned*>(sub);
+    for (int e = tid; e < S3; e += nthr) {
+        _Float16 hi, lo;
+        split_f16(ldexpf(sub[e], sg), hi, lo);
+        subp[e] = (unsigned)__builtin_bit_cast(unsigned short, hi) |
+                  ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
+    }
+    __syncthreads();
 
     // equal shares of the item's records, one per wave
-    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nthr >> 6;
     const int rb = beg + (int)(((long)n * wave) / nw), re = beg + (int)(((long)n * (wave + 1)) / nw);
     const int j = lane & 15, q = lane >> 4;
     const float h = (float)g.h, c = (float)g.c, dn = (float)g.d;
     const float kc = -c * 1.4426950408889634f;
     const float fh = (float)g.first * h;
     const double h3 = g.h * g.h * g.h;
+    // z factors enter the matrix instructions as fp16 pairs of Fz 2^15 (Fz <= 1) and of
+    // Fz' 2^sd (|Fz'| = c |a| exp(-c a^2) <= sqrt(c / 2e))
+    const int sd = 14 - ilogbf(sqrtf(c * 0.18393972058572117f));
 
     // Groups: runs of up to 16 consecutive records of one column; lane j + 16 q holds record
     // g0 + j (slots past the run repeat its last record and store nothing).  The next
@@ -418,15 +473,23 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
             lon = rec[sn].lo;
             orn = rec[sn].orig;
         }
-        // B operands: particle j's z factors at window points 4 q + s, s = 0..3
-        float bz[4], bzd[4];
+        // B operands: particle j's z factors at window points 4 q + s, s = 0..3, as fp16 pairs
+        v4h bh, bl, bdh, bdl;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const float az = fmaf(-(float)(4 * q + s), h, r.z - fh);
-            bz[s] = gauss_c(kc, az);
-            if (FORCE) bzd[s] = bz[s] * (-c * az);
+            const float bz = gauss_c(kc, az);
+            _Float16 hi, lo;
+            split_f16(bz * 32768.f, hi, lo);
+            bh[s] = hi;
+            bl[s] = lo;
+            if (FORCE) {
+                split_f16(ldexpf(bz * (-c * az), sd), hi, lo);
+                bdh[s] = hi;
+                bdl[s] = lo;
+            }
         }
-        const float* ap = sub + (x0 + j) * S + z0 + 4 * q;
+        const unsigned* ap = subp + (x0 + j) * S + z0 + 4 * q;
         const float tyf = r.y - fh;   // y offset of window index w: tyf - w h
         const v4f zero4 = {0.f, 0.f, 0.f, 0.f};
         v4f U = zero4, V = zero4, W = zero4;
@@ -436,19 +499,20 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
         // row S - 1 again (in the sub-grid) and is not used.
         v4f Tp = zero4, Tdp = zero4;
         float fyp = 0.f, fydp = 0.f;
-        const float* a = ap + y0 * S2;
-        float a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+        const unsigned* a = ap + y0 * S2;
+        unsigned a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
         for (int y = y0; y < y1; ++y) {
-            v4f T = mfma16(a0, bz[0], zero4);
-            T = mfma16(a1, bz[1], T);
-            T = mfma16(a2, bz[2], T);
-            T = mfma16(a3, bz[3], T);
+            // G Fz = hi hi + lo hi + hi lo (lo lo, about 2^-22 of the product, left out);
+            // the small terms first
+            const v4h ah = pack_hi(a0, a1, a2, a3), al = pack_lo(a0, a1, a2, a3);
+            v4f T = mfma16h(al, bh, zero4);
+            T = mfma16h(ah, bl, T);
+            T = mfma16h(ah, bh, T);
             v4f Td = zero4;
             if (FORCE) {
-                Td = mfma16(a0, bzd[0], zero4);
-                Td = mfma16(a1, bzd[1], Td);
-                Td = mfma16(a2, bzd[2], Td);
-                Td = mfma16(a3, bzd[3], Td);
+                Td = mfma16h(al, bdh, zero4);
+                Td = mfma16h(ah, bdl, Td);
+                Td = mfma16h(ah, bdh, Td);
             }
             a = ap + min(y + 1, S - 1) * S2;
             a0 = a[0];
@@ -491,11 +555,12 @@ __global__ __launch_bounds__(1024) void k_interp_col16(const Rec<float>* __restr
                 sz = fmaf(fx, W[i], sz);
             }
         }
-        if (POT) sp = sum_rows4(sp);
+        // undo the operand scales (powers of two: exact)
+        if (POT) sp = ldexpf(sum_rows4(sp), -(sg + 15));
         if (FORCE) {
-            sx = sum_rows4(sx);
-            sy = sum_rows4(sy);
-            sz = sum_rows4(sz);
+            sx = ldexpf(sum_rows4(sx), -(sg + 15));
+            sy = ldexpf(sum_rows4(sy), -(sg + 15));
+            sz = ldexpf(sum_rows4(sz), -(sg + sd));
         }
         // Wait for the next group's records here, before this group's output stores: vmcnt
         // counts loads and stores together, so a wait placed after the stores (at the top of
