@@ -365,7 +365,7 @@ bool launch_interp_pencil(const Rec<float>* rec, const int* item_tile, const int
 // column spread sums; the other path with matrix cores (the MFMA gather) goes by size.
 constexpr int COLUMN_MAX = 2048;                  // column items are staged in LDS: at most this many records
 constexpr int COLUMN_NONE = -2147483647 - 1;      // item_col entry of an item the column kernels leave
-constexpr float COLUMN_GROUP_FILL = 5.5f;         // column items: at least this many particles per column group
+constexpr float COLUMN_GROUP_FILL = 3.5f;         // column items: at least this many particles per column group
 struct DenseSel {
     const int* col;   // per-item column table (column path) or nullptr
     int min;          // without a table: dense from min to COLUMN_MAX particles (0: none)
