@@ -61,9 +61,13 @@ __device__ __forceinline__ v4f mfma16h(v4h a, v4h b, v4f c)
 }
 
 // fp16 pair of v: hi = fp16(v), lo = fp16(v - hi), so that hi + lo holds v to about 2^-22
-// of |v| (or 2^-25 absolute where lo is subnormal)
+// of |v| (or 2^-25 absolute where lo is subnormal).  v is pinned as an FP32 value first:
+// otherwise the compiler may fuse the product that forms v into the lo conversion
+// (v_fma_mix, exact product) while forming hi from the rounded product, and where the two
+// round to different fp16 values hi + lo is off by an fp16 ulp of v.
 __device__ __forceinline__ void split_f16(float v, _Float16& hi, _Float16& lo)
 {
+    asm volatile("" : "+v"(v));
     hi = (_Float16)v;
     lo = (_Float16)(v - (float)hi);
 }
