@@ -465,13 +465,15 @@ __global__ __launch_bounds__(1024, 8) void k_spread_plane(const Rec<float>* __re
                                                           const int* __restrict__ item_end,
                                                           const int* __restrict__ n_items,
                                                           const int* __restrict__ tile_start, Geom g, Tiling tl,
-                                                          float* __restrict__ partials, int sparse_max)
+                                                          float* __restrict__ partials, int sparse_max,
+                                                          DenseSel dense)
 {
     constexpr int SM = 12 + P - 1;   // largest sub-grid side (tile side <= 12)
     static_assert(SM <= 32 && P <= 16, "the y table holds 32 rows, the x and z tables 16 + 16 + 16");
     const int item = blockIdx.x;
     if (item >= *n_items) return;
     if (!sparse_item(item_tile, item_beg, item_end, tile_start, item, sparse_max)) return;
+    if (dense(item, item_end[item] - item_beg[item])) return;   // a column item (the column spread's)
     __shared__ float tx[PLANE_BATCH][PLANE_TW], tz[PLANE_BATCH][PLANE_TW], ty[PLANE_BATCH][32];
     const int beg = item_beg[item], end = item_end[item];
     const int S = tl.S, tid = threadIdx.x, nthr = blockDim.x;
@@ -1197,7 +1199,7 @@ bool launch_spread_pencil(const Rec<float>* rec, const int* item_tile, const int
                            partials, item_tile, tile_start, sparse_max, dense);                               \
         if (sparse_max > 0)                                                                                  \
             hipLaunchKernelGGL(k_spread_plane<PV>, gr, dim3(64 * ((tl.S + 1) / 2)), 0, sv, rec, item_tile,     \
-                               item_beg, item_end, n_items, tile_start, g, tl, partials, sparse_max);         \
+                               item_beg, item_end, n_items, tile_start, g, tl, partials, sparse_max, dense);  \
     } while (0)
     switch (g.P) {
     case 8: SE_SP(8); break;
