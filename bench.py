@@ -43,72 +43,35 @@ HBM_PEAK_GBS = 8000.0
 #   fma_insts     = SQ_INSTS_VALU_FMA_F32 (wave instructions, 64 lanes each) of all VALU
 #                   instructions SQ_INSTS_VALU (which counts the MFMAs, SQ_INSTS_MFMA, too)
 # The spread and the gather of the clustered workload are several kernels each (spread:
-# column spread on the dense items, pencil register ring on the medium ones, plane spread on
-# the sparse tiles, one after the other; gather: column gather on the dense items and row
-# gather on the others, concurrently on two streams); each is listed under its stage.
+# column spread on the column items, beside the pencil register ring on the medium items and
+# the plane spread on the sparse tiles on a second stream; gather: column gather on the
+# column items and row gather on the others, concurrently on two streams); each is listed
+# under its stage.
 MEASURED = {
     ("clustered", 4_000_000, 16, 256, "fp32", 12): {
-        "spread": {"traffic_bytes": 285.3e6 + 41.6e6 + 832.0e6,
-                   "k_spread_col16": {"traffic_bytes": 285.3e6, "valu_busy_pct": 22.1, "mfma_util_pct": 25.8,
-                                      "valu_insts": 1.04e8, "mfma_insts": 1.57e7, "lds_bank_conflicts": 1.45e6,
+        "spread": {"traffic_bytes": 303.5e6 + 23.3e6 + 832.0e6,
+                   "k_spread_col16": {"traffic_bytes": 303.5e6, "valu_busy_pct": 27.7, "mfma_util_pct": 33.3,
+                                      "valu_insts": 1.05e8, "mfma_insts": 1.64e7, "lds_bank_conflicts": 1.17e7,
                                       "lds_insts": 1.07e7},
-                   "k_spread_pencil": {"traffic_bytes": 41.6e6, "valu_busy_pct": 19.2, "valu_insts": 3.22e7,
-                                       "fma_insts": 1.67e7, "lds_bank_conflicts": 2.48e5, "lds_insts": 2.94e6},
-                   "k_spread_plane": {"traffic_bytes": 832.0e6, "valu_busy_pct": 79.5, "valu_insts": 2.64e8,
+                   "k_spread_pencil": {"traffic_bytes": 23.3e6, "valu_busy_pct": 12.3, "valu_insts": 1.27e7,
+                                       "fma_insts": 5.5e6, "lds_bank_conflicts": 1.38e5, "lds_insts": 1.35e6},
+                   "k_spread_plane": {"traffic_bytes": 832.0e6, "valu_busy_pct": 79.2, "valu_insts": 2.64e8,
                                       "fma_insts": 1.78e8, "lds_bank_conflicts": 7.7e5, "lds_insts": 2.13e7}},
-        "gather": {"traffic_bytes": 646.5e6 + 1281.9e6,
-                   "k_interp_col16": {"traffic_bytes": 646.5e6, "mfma_util_pct": 37.6, "valu_busy_pct": 30.2,
-                                      "valu_insts": 1.76e8, "mfma_insts": 2.84e7, "fma_insts": 5.39e7,
-                                      "lds_bank_conflicts": 2.84e7, "lds_insts": 7.11e6},
-                   "k_interp_row16": {"traffic_bytes": 1281.9e6, "valu_busy_pct": 70.8, "valu_insts": 3.0e8,
-                                      "fma_insts": 1.4e8, "lds_bank_conflicts": 1.12e8, "lds_insts": 3.18e7}},
+        "gather": {"traffic_bytes": 665.5e6 + 1252.5e6,
+                   "k_interp_col16": {"traffic_bytes": 665.5e6, "mfma_util_pct": 23.9, "valu_busy_pct": 59.4,
+                                      "valu_insts": 2.19e8, "mfma_insts": 2.3e7, "fma_insts": 5.8e7,
+                                      "lds_bank_conflicts": 3.06e7, "lds_insts": 1.02e7},
+                   "k_interp_row16": {"traffic_bytes": 1252.5e6, "valu_busy_pct": 80.7, "valu_insts": 2.75e8,
+                                      "fma_insts": 1.21e8, "lds_bank_conflicts": 9.91e7, "lds_insts": 2.74e7}},
     },
     ("uniform", 1_000_000, 16, 128, "fp32", 12): {
-        "spread": {"traffic_bytes": 137.3e6,
-                   "k_spread_pencil": {"traffic_bytes": 137.3e6, "valu_busy_pct": 64.5, "valu_insts": 1.57e8,
+        "spread": {"traffic_bytes": 137.4e6,
+                   "k_spread_pencil": {"traffic_bytes": 137.4e6, "valu_busy_pct": 64.9, "valu_insts": 1.57e8,
                                        "fma_insts": 8.21e7, "lds_bank_conflicts": 8.19e5, "lds_insts": 1.51e7}},
-        "gather": {"traffic_bytes": 280.9e6,
-                   "k_interp_row16": {"traffic_bytes": 280.9e6, "valu_busy_pct": 70.1, "valu_insts": 2.05e8,
+        "gather": {"traffic_bytes": 277.7e6,
+                   "k_interp_row16": {"traffic_bytes": 277.7e6, "valu_busy_pct": 71.5, "valu_insts": 2.07e8,
                                       "fma_insts": 1.41e8, "lds_bank_conflicts": 1.1e8, "lds_insts": 3.21e7}},
     },
-}
-
-
-def make_system(N, seed=2):
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    x = rng.random(3 * N)
-    q = rng.uniform(-1.0, 1.0, N)
-    q -= q.mean()
-    return x, q
-
-
-def make_clustered(N, seed=3, n_clusters=64, frac=0.8, sigma=0.01):
-    """80 % of the charges in 64 isotropic Gaussian clusters (sigma = 0.01 L, centres
-    uniform, wrapped into the box), 20 % uniform; q = +-1 in exact halves, shuffled."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    nc = int(frac * N)
-    centres = rng.random((3, n_clusters))
-    lab = rng.integers(0, n_clusters, nc)
-    xc = (centres[:, lab] + sigma * rng.standard_normal((3, nc))) % 1.0
-    xu = rng.random((3, N - nc))
-    x = np.concatenate([xc, xu], axis=1)
-    x = np.where(x >= 1.0, 0.0, x)   # (-tiny) % 1.0 can round to 1.0
-    q = np.ones(N)
-    q[N // 2:] = -1.0
-    if N % 2:
-        q[-1] = 0.0
-    rng.shuffle(q)
-    return np.ascontiguousarray(x.reshape(-1)), q
-
-
-WORKLOADS = {
-    # name: (N, M, P, xi, generator, description)
-    "clustered": (4_000_000, 256, 16, 80.0, make_clustered,
-                  "N={N}: 80% in 64 Gaussian clusters (sigma=0.01 L), 20% uniform, q=+-1 net zero"),
-    "uniform": (1_000_000, 128, 16, 40.0, make_system,
-                "N={N} uniform charges in L=1, charges uniform in [-1,1] shifted to zero sum"),
 }
 
 
