@@ -75,6 +75,44 @@ MEASURED = {
 }
 
 
+def make_system(N, seed=2):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    x = rng.random(3 * N)
+    q = rng.uniform(-1.0, 1.0, N)
+    q -= q.mean()
+    return x, q
+
+
+def make_clustered(N, seed=3, n_clusters=64, frac=0.8, sigma=0.01):
+    """80 % of the charges in 64 isotropic Gaussian clusters (sigma = 0.01 L, centres
+    uniform, wrapped into the box), 20 % uniform; q = +-1 in exact halves, shuffled."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    nc = int(frac * N)
+    centres = rng.random((3, n_clusters))
+    lab = rng.integers(0, n_clusters, nc)
+    xc = (centres[:, lab] + sigma * rng.standard_normal((3, nc))) % 1.0
+    xu = rng.random((3, N - nc))
+    x = np.concatenate([xc, xu], axis=1)
+    x = np.where(x >= 1.0, 0.0, x)   # (-tiny) % 1.0 can round to 1.0
+    q = np.ones(N)
+    q[N // 2:] = -1.0
+    if N % 2:
+        q[-1] = 0.0
+    rng.shuffle(q)
+    return np.ascontiguousarray(x.reshape(-1)), q
+
+
+WORKLOADS = {
+    # name: (N, M, P, xi, generator, description)
+    "clustered": (4_000_000, 256, 16, 80.0, make_clustered,
+                  "N={N}: 80% in 64 Gaussian clusters (sigma=0.01 L), 20% uniform, q=+-1 net zero"),
+    "uniform": (1_000_000, 128, 16, 40.0, make_system,
+                "N={N} uniform charges in L=1, charges uniform in [-1,1] shifted to zero sum"),
+}
+
+
 def _cpu_model():
     try:
         for ln in open("/proc/cpuinfo"):
