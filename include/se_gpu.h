@@ -57,7 +57,7 @@ typedef struct {
     int chunk;         /* max particles per work item, 0 = automatic */
     int block;         /* threads per work-group (<= 512) of every kernel (the P = 16 MFMA gather
                           uses at most 256; the P = 16 column spread and gather always run
-                          768 and 512); 0 = each kernel's own default: 768 for the FP32
+                          512); 0 = each kernel's own default: 768 for the FP32
                           register-ring spread, 256 for the MFMA gather, 512 otherwise */
     int flags;         /* SE_PLAN_* bits */
     double slab;       /* SE2P: particles have 0 <= z < slab; SE1P: particles have
@@ -80,7 +80,10 @@ typedef struct {
 #define SE_PLAN_NO_PLANE_SPREAD 4
 /* Run the dense work items of the P = 16 FP32 path on the 4 x 1 pencil kernels instead of
  * the 1 x 1 column kernels, whose matrix-core products take sixteen particles of one
- * window-start column at a time (for A/B comparisons and tests). */
+ * window-start column at a time (for A/B comparisons and tests).  Without this flag the
+ * column kernels take each work item whose column groups hold 3.5 particles or more on
+ * average (decided on the device per item); the column gather forms its products from
+ * fp16 (hi, lo) pairs of the FP32 operands, good to about 2^-22 each. */
 #define SE_PLAN_NO_COLUMN 8
 
 int se_plan_create(se_plan** plan, const se_plan_desc* desc, void* hip_stream);
