@@ -92,11 +92,12 @@ static Tiling choose_tiling(int esize, int P, int M, int Mz, int n_cap, int n_sy
     tl.K = tl.PX * TS * TS;
     SE_REQUIRE((long)tl.ntiles * tl.K < (1l << 30), SE_ERR_ARG, "grid too large for the sort keys");
     tl.nbins = tl.ntiles * tl.K;
-    // Default work-item cap: heavy tiles are cut into items of 1280 particles.  Clustered
-    // N = 4M at M = 256 (updates/s): 1024 6.85e12, 1280 7.06e12, 1536 6.96e12, 1792 6.94e12,
-    // 2048 6.78e12, 3072 6.71e12 -- smaller items balance the one-work-group-per-CU spread
-    // better until their extra partials cost the merge more than that.
-    tl.chunk = chunk > 0 ? chunk : 1280;
+    // Default work-item cap: heavy tiles are cut into items of 1664 particles.  Clustered
+    // N = 4M at M = 256 (updates/s, with the column kernels): 1024 9.75e12, 1280 1.005e13,
+    // 1536 1.023e13, 1664 1.027e13, 1792 1.026e13, 2048 1.009e13 -- larger items write fewer
+    // partials for the merge (0.43 -> 0.40 ms from 1280 to 1664) until the spread's items
+    // balance worse over the CUs.
+    tl.chunk = chunk > 0 ? chunk : 1664;
     tl.block = block > 0 ? block : 512;
     SE_REQUIRE(tl.block % 64 == 0 && tl.block <= 512, SE_ERR_ARG, "block must be a multiple of 64, <= 512");
     tl.sub_elems = tl.S * tl.S * tl.Szp;
