@@ -304,6 +304,35 @@ def test_column_kernels_match_pencil_kernels(oracle, clustered):
         assert rms_rel(a[0], ref_phi) < FP32_TOL and rms_rel(a[1], ref_F) < FP32_TOL_F, kw
 
 
+def test_column_items_are_chosen_by_group_fill():
+    """Work items go to the column kernels only where their column groups (runs of up to
+    16 particles of one 1 x 1 column within five rows) hold 3.5 particles or more on
+    average: a uniform system at 0.46 particles per grid cell has none (its results equal
+    the SE_PLAN_NO_COLUMN ones bit for bit), the same particles packed into an eighth of
+    the box (3.7 per cell) have column items (the results differ in the last bits: the
+    column spread sums in 32-bit fixed point)."""
+    N, M, P, xi = 15000, 32, 16, 8.0
+    rng = np.random.default_rng(77)
+    q = rng.uniform(-1, 1, N)
+    q -= q.mean()
+    qt = torch.tensor(q, device=DEV)
+    for scale, expect_equal in ((1.0, True), (0.5, False)):
+        x = rng.random((3, N)) * scale
+        xt = torch.tensor(np.ascontiguousarray(x.reshape(-1)), device=DEV)
+        out = {}
+        for col in (True, False):
+            plan = se.Plan(M, P, 1.0, xi, N, precision=se.PREC_FP32, column=col)
+            phi = torch.zeros(N, dtype=torch.float64, device=DEV)
+            plan.solve(xt, qt, phi)
+            torch.cuda.synchronize()
+            plan.close()
+            out[col] = phi.cpu().numpy()
+        same = np.array_equal(out[True], out[False])
+        print("scale %.2f: column path %s" % (scale, "not taken" if same else "taken"), flush=True)
+        assert same == expect_equal, scale
+        assert rms_rel(out[True], out[False]) < 2e-6
+
+
 @pytest.mark.parametrize("clustered", [False, True])
 def test_row_gather_of_sparse_items_matches_vector_gather(oracle, clustered):
     """Sparse work items at P = 16 (fewer particles than the matrix-core threshold) take
