@@ -445,9 +445,20 @@ void launch_spread_tiles(const Rec<T>* rec, const int* item_tile, const int* ite
 // keeps their sums in registers, so the (box, item) pairs need no barrier between them:
 // the thread walks the pairs in order (MB at a time with their loads in flight; one
 // measured fastest, as more registers per thread cost occupancy), and adds each value to
-// its point in that order, so the result does not depend on scheduling.  FP32 partials are summed in FP64; FP64 partials with Kahan
-// compensation.
+// its point in that order, so the result does not depend on scheduling.  FP32 partials
+// are summed in FP64; FP64 partials with Kahan compensation.
 // seg[t * sstride + a] = (source tile coordinate, local start, output start, length).
+//
+// Addressing: point (x, y, z) of the tile lies at lin = (x S + y) S + z plus a shift
+// that is fixed per box in the partial sub-grid of a source item, so each box costs one
+// scalar base and each point a 32-bit offset of a raw buffer load.  A point outside the
+// box takes an offset past the buffer's range, which loads 0 without a memory access:
+// the loads need no branch, and an FP32 value of 0 is added unconditionally (adding +0
+// leaves an FP64 sum that starts at +0 bit for bit unchanged).  With MASK, a point's
+// membership of the tile's segments is one word (bit a: x segment a, bit 10 + b: y
+// segment b, bit 20 + c: z segment c), so the box test is one AND and compare against
+// the box's three bits; without it (more than 10 segments per dimension, only on small
+// grids) the word holds the point's coordinates and the test reads the box's segments.
 // ================================================================================
 template <typename T>
 __device__ __forceinline__ void merge_add(double& acc, double& cmp, T v);
@@ -465,16 +476,18 @@ __device__ __forceinline__ void merge_add<double>(double& acc, double& cmp, doub
     acc = t;
 }
 
-// LDS of the merge: the segment and item-range tables (3 sstride int4 + sstride^3 int2)
+// LDS of the merge: the segment tables and, per segment triple, (item range, shift, bits)
 inline size_t merge_lds_bytes(int sstride)
 {
-    return (size_t)3 * sstride * sizeof(int4) + (size_t)sstride * sstride * sstride * sizeof(int2);
+    return (size_t)3 * sstride * sizeof(int4) + (size_t)sstride * sstride * sstride * sizeof(int4);
 }
 
-constexpr int MERGE_THREADS = 256;   // threads per work-group (one tile)
+constexpr int MERGE_THREADS = 256;        // threads per work-group (one tile)
+constexpr int MERGE_MASK_SEGS = 10;       // most segments per dimension of the MASK form
+constexpr int MERGE_RANGE = 0x7ffffff0;   // byte range of the merge's buffer loads
 
 // JM tile points per thread (TS^3 <= MERGE_THREADS JM); MB (box, item) pairs in flight
-template <typename T, int JM, int MB>
+template <typename T, int JM, int MB, bool MASK>
 __global__ __launch_bounds__(MERGE_THREADS) void k_merge_tiles(const T* __restrict__ partials,
                                                      const int* __restrict__ chunk_start,
                                                      const int4* __restrict__ seg, const int* __restrict__ nseg,
@@ -498,13 +511,13 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_merge_tiles(const T* __restri
     const int nx = min(TS, M - ox), ny = min(TS, M - oy), nz = min(TS, Mz - oz);
     const int V = nx * ny * nz;
     // the tile's source boxes: the segments of each dimension and, per segment triple,
-    // the item range of its source tile, fetched in parallel into LDS once instead of one
-    // dependent global load per triple
+    // the item range of its source tile, the box's shift and its segment bits, fetched in
+    // parallel into LDS once instead of one dependent global load per triple
     extern __shared__ __align__(16) unsigned char smem_raw[];
     int4* sA = reinterpret_cast<int4*>(smem_raw);
     int4* sB = sA + sstride;
     int4* sC = sB + sstride;
-    int2* sR = reinterpret_cast<int2*>(sC + sstride);
+    int4* sR = sC + sstride;
     const int na = nseg[tx], nb = nseg[ty], nc = nsegz[tz], ntri = na * nb * nc;
     if (threadIdx.x < na) sA[threadIdx.x] = seg[tx * sstride + threadIdx.x];
     if (threadIdx.x < nb) sB[threadIdx.x] = seg[ty * sstride + threadIdx.x];
@@ -512,27 +525,43 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_merge_tiles(const T* __restri
     __syncthreads();
     for (int t = threadIdx.x; t < ntri; t += MERGE_THREADS) {
         const int a = t / (nb * nc), b = (t / nc) % nb, c = t % nc;
-        const int src = (sA[a].x * NT + sB[b].x) * NTz + sC[c].x;
-        sR[t] = make_int2(chunk_start[src], chunk_start[src + 1]);
+        const int4 A = sA[a], B = sB[b], C = sC[c];
+        const int src = (A.x * NT + B.x) * NTz + C.x;
+        const unsigned bits = MASK ? (1u << a) | (1u << (10 + b)) | (1u << (20 + c)) : 0u;
+        sR[t] = make_int4(chunk_start[src], chunk_start[src + 1], ((A.y - A.z) * S + (B.y - B.z)) * S + (C.y - C.z),
+                          (int)bits);
     }
     __syncthreads();
-    // owned points, packed (x << 20 | y << 10 | z); -1 past the tile
-    int pt[JM];
+    // per owned point: its offset lin and its word (segment bits with MASK, else the packed
+    // coordinates x << 20 | y << 10 | z, all ones past the tile)
+    int lin[JM];
+    unsigned pk[JM];
     double acc[JM], cmp[JM];
 #pragma unroll
     for (int j = 0; j < JM; ++j) {
         const int e = threadIdx.x + MERGE_THREADS * j;
         const int ez = e % nz, ey = (e / nz) % ny, ex = e / (nz * ny);
-        pt[j] = e < V ? (ex << 20) | (ey << 10) | ez : -1;
+        unsigned w = MASK ? 0u : ~0u;
+        if (e < V) {
+            if constexpr (MASK) {
+                for (int a = 0; a < na; ++a) w |= (unsigned)((unsigned)(ex - sA[a].z) < (unsigned)sA[a].w) << a;
+                for (int b = 0; b < nb; ++b) w |= (unsigned)((unsigned)(ey - sB[b].z) < (unsigned)sB[b].w) << (10 + b);
+                for (int c = 0; c < nc; ++c) w |= (unsigned)((unsigned)(ez - sC[c].z) < (unsigned)sC[c].w) << (20 + c);
+            } else {
+                w = ((unsigned)ex << 20) | ((unsigned)ey << 10) | (unsigned)ez;
+            }
+        }
+        pk[j] = w;
+        lin[j] = (ex * S + ey) * S + ez;
         acc[j] = 0.0;
         cmp[j] = 0.0;
     }
-    // cursor over the (box, item) pairs in order: box t, item it of [it, ie)
+    // cursor over the (box, item) pairs in order: box t, item it of [it, ie), all uniform
     int t = 0, it = 0, ie = 0;
     while (t < ntri && sR[t].x == sR[t].y) ++t;
     if (t < ntri) {
-        it = sR[t].x;
-        ie = sR[t].y;
+        it = __builtin_amdgcn_readfirstlane(sR[t].x);
+        ie = __builtin_amdgcn_readfirstlane(sR[t].y);
     }
     while (t < ntri) {
         T v[MB][JM];
@@ -541,24 +570,42 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_merge_tiles(const T* __restri
         for (int m = 0; m < MB; ++m) {
             msk[m] = 0;
             if (t < ntri) {
-                const int a = t / (nb * nc), b = (t / nc) % nb, c = t % nc;
-                const int4 A = sA[a], B = sB[b], C = sC[c];
-                const T* base = partials + (size_t)it * tl.part_elems;
+                const int shift = __builtin_amdgcn_readfirstlane(sR[t].z);
+                const T* base = partials + (long)it * tl.part_elems + shift;
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, MERGE_RANGE, 0x00020000);
+                unsigned sel = 0;
+                int4 A{}, B{}, C{};
+                if constexpr (MASK) {
+                    sel = (unsigned)__builtin_amdgcn_readfirstlane(sR[t].w);
+                } else {
+                    A = sA[t / (nb * nc)];
+                    B = sB[(t / nc) % nb];
+                    C = sC[t % nc];
+                }
 #pragma unroll
                 for (int j = 0; j < JM; ++j) {
-                    const int px = pt[j] >> 20, py = (pt[j] >> 10) & 1023, pz = pt[j] & 1023;
-                    const int dx = px - A.z, dy = py - B.z, dz = pz - C.z;
-                    const bool in = pt[j] >= 0 && (unsigned)dx < (unsigned)A.w && (unsigned)dy < (unsigned)B.w &&
-                                    (unsigned)dz < (unsigned)C.w;
-                    v[m][j] = in ? base[((A.y + dx) * S + (B.y + dy)) * S + C.y + dz] : (T)0;
+                    bool in;
+                    if constexpr (MASK) {
+                        in = (pk[j] & sel) == sel;
+                    } else {
+                        in = pk[j] != ~0u && (pk[j] >> 20) - (unsigned)A.z < (unsigned)A.w &&
+                             ((pk[j] >> 10) & 1023u) - (unsigned)B.z < (unsigned)B.w &&
+                             (pk[j] & 1023u) - (unsigned)C.z < (unsigned)C.w;
+                    }
+                    const int off = in ? lin[j] * (int)sizeof(T) : MERGE_RANGE;
+                    if constexpr (sizeof(T) == 4)
+                        v[m][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+                    else
+                        v[m][j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
                     msk[m] |= (unsigned)in << j;
                 }
                 if (++it == ie) {
                     ++t;
                     while (t < ntri && sR[t].x == sR[t].y) ++t;
                     if (t < ntri) {
-                        it = sR[t].x;
-                        ie = sR[t].y;
+                        it = __builtin_amdgcn_readfirstlane(sR[t].x);
+                        ie = __builtin_amdgcn_readfirstlane(sR[t].y);
                     }
                 }
             }
@@ -567,13 +614,14 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_merge_tiles(const T* __restri
         for (int m = 0; m < MB; ++m)
 #pragma unroll
             for (int j = 0; j < JM; ++j)
-                if (msk[m] >> j & 1u) merge_add<T>(acc[j], cmp[j], v[m][j]);
+                if (sizeof(T) == 4 || (msk[m] >> j & 1u)) merge_add<T>(acc[j], cmp[j], v[m][j]);
     }
 #pragma unroll
-    for (int j = 0; j < JM; ++j)
-        if (pt[j] >= 0)
-            grid[((long)(ox + (pt[j] >> 20)) * M + (oy + ((pt[j] >> 10) & 1023))) * Mz + (oz + (pt[j] & 1023))] =
-                (T)acc[j];
+    for (int j = 0; j < JM; ++j) {
+        const int e = threadIdx.x + MERGE_THREADS * j;
+        const int ez = e % nz, ey = (e / nz) % ny, ex = e / (nz * ny);
+        if (e < V) grid[((long)(ox + ex) * M + (oy + ey)) * Mz + (oz + ez)] = (T)acc[j];
+    }
 }
 
 template <typename T, int JM>
@@ -585,9 +633,15 @@ static void launch_merge_jm(const T* partials, const int* chunk_start, const int
     const int cols8 = (tl.NT * tl.NT + 7) / 8;
     dim3 gr(8 * cols8 * tl.NTz, nsys), bl(MERGE_THREADS);
     const size_t lds = merge_lds_bytes(sstride);
-    allow_lds((const void*)k_merge_tiles<T, JM, MB>, lds);
-    hipLaunchKernelGGL((k_merge_tiles<T, JM, MB>), gr, bl, lds, s, partials, chunk_start, seg, nseg, segz, nsegz,
-                       sstride, g, tl, grid);
+    if (sstride <= MERGE_MASK_SEGS) {
+        allow_lds((const void*)k_merge_tiles<T, JM, MB, true>, lds);
+        hipLaunchKernelGGL((k_merge_tiles<T, JM, MB, true>), gr, bl, lds, s, partials, chunk_start, seg, nseg, segz,
+                           nsegz, sstride, g, tl, grid);
+    } else {
+        allow_lds((const void*)k_merge_tiles<T, JM, MB, false>, lds);
+        hipLaunchKernelGGL((k_merge_tiles<T, JM, MB, false>), gr, bl, lds, s, partials, chunk_start, seg, nseg, segz,
+                           nsegz, sstride, g, tl, grid);
+    }
 }
 
 template <typename T>

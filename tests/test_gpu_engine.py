@@ -196,6 +196,19 @@ def test_tiling_and_chunking_invariance(prec):
     assert solve(x, q, M, P, xi, prec, chunk=16)[2]["items"] > info0["items"]
 
 
+@pytest.mark.parametrize("prec", [se.PREC_FP64, se.PREC_FP32])
+def test_merge_with_many_segments_per_dimension(prec):
+    """Tile side 1 at P = 16 (16 source runs per output tile and dimension, past the merge's
+    10-segment bit word, so it tests every box's segments) gives the default tiling's answer."""
+    N, M, P, xi = 1500, 24, 16, 5.0
+    x, q = system(N, 17, clustered=True)
+    ref, Fref, _ = solve(x, q, M, P, xi, prec)
+    tol = 1e-12 if prec == se.PREC_FP64 else 2e-6
+    for kw in (dict(tile=1), dict(tile=2, chunk=50)):
+        phi, F, _ = solve(x, q, M, P, xi, prec, **kw)
+        assert rms_rel(phi, ref) < tol and rms_rel(F, Fref) < tol, kw
+
+
 @pytest.mark.parametrize("P", [8, 11, 16])
 def test_pencil_kernels_match_oracle_and_generic(oracle, P):
     """FP32 register-ring (pencil) kernels vs the FP64 oracle and vs the generic kernels,
