@@ -165,6 +165,30 @@ __device__ __forceinline__ void load_tile_dma(T* sub, const T* __restrict__ grid
     const int S3 = S * S * S, lane = tid & 63, nw = nthr >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     SubWalk w(S, EPW * wave + lane / DPE, EPW * nw);   // (a, b) = (iy, ix) if YXZ, else (ix, iy); c = iz
+    if (ox + S <= M && oy + S <= M && oz + S <= Mz && (long)M * M * Mz < (1l << 31)) {
+        // an interior tile (no periodic wrap): the point's element offset from the tile's
+        // origin is stepped with the walk's digits, a carry out of digit c or b adding
+        // the stride difference it makes, so a load costs no multiply and no wrap test
+        const T* tb = grid + ((long)ox * M + oy) * Mz + oz;
+        const int sA = YXZ ? Mz : M * Mz, sB = YXZ ? M * Mz : Mz;   // strides of digits a and b
+        int off = w.a * sA + w.b * sB + w.c;
+        const int d0 = w.da * sA + w.db * sB + w.dc, cB = sB - S, cA = sA - S * sB;
+        const int last = (S - 1) * (sA + sB + 1);
+        for (int e0 = EPW * wave; e0 < S3; e0 += EPW * nw) {
+            const int o = e0 + lane / DPE < S3 ? off : last;
+            const unsigned* src = reinterpret_cast<const unsigned*>(tb + o) + lane % DPE;
+            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(sub + e0), 4, 0, 0);
+            w.c += w.dc;
+            const bool k1 = w.c >= S;
+            w.c -= k1 ? S : 0;
+            w.b += w.db + (k1 ? 1 : 0);
+            const bool k2 = w.b >= S;
+            w.b -= k2 ? S : 0;
+            off += d0 + (k1 ? cB : 0) + (k2 ? cA : 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): every DMA write has landed in LDS
+        return;
+    }
     for (int e0 = EPW * wave; e0 < S3; e0 += EPW * nw) {
         const bool in = e0 + lane / DPE < S3;
         const int a = in ? w.a : S - 1, b = in ? w.b : S - 1, c = in ? w.c : S - 1;
