@@ -49,19 +49,19 @@ HBM_PEAK_GBS = 8000.0
 # under its stage.
 MEASURED = {
     ("clustered", 4_000_000, 16, 256, "fp32", 12): {
-        "spread": {"traffic_bytes": 261.2e6 + 20.7e6 + 832.6e6,
-                   "k_spread_col16": {"traffic_bytes": 261.2e6, "valu_busy_pct": 25.9, "mfma_util_pct": 32.8,
+        "spread": {"traffic_bytes": 261.4e6 + 20.7e6 + 832.6e6,
+                   "k_spread_col16": {"traffic_bytes": 261.4e6, "valu_busy_pct": 25.0, "mfma_util_pct": 31.7,
                                       "valu_insts": 1.0e8, "mfma_insts": 1.64e7, "lds_bank_conflicts": 1.05e7,
                                       "lds_insts": 9.99e6},
                    "k_spread_pencil": {"traffic_bytes": 20.7e6, "valu_busy_pct": 12.8, "valu_insts": 1.14e7,
                                        "fma_insts": 4.93e6, "lds_bank_conflicts": 1.2e5, "lds_insts": 1.23e6},
-                   "k_spread_plane": {"traffic_bytes": 832.6e6, "valu_busy_pct": 78.6, "valu_insts": 2.64e8,
+                   "k_spread_plane": {"traffic_bytes": 832.6e6, "valu_busy_pct": 78.4, "valu_insts": 2.64e8,
                                       "fma_insts": 1.78e8, "lds_bank_conflicts": 7.7e5, "lds_insts": 2.13e7}},
-        "gather": {"traffic_bytes": 663.9e6 + 1248.0e6,
-                   "k_interp_col16": {"traffic_bytes": 663.9e6, "mfma_util_pct": 24.5, "valu_busy_pct": 58.4,
-                                      "valu_insts": 2.1e8, "mfma_insts": 2.29e7, "fma_insts": 5.78e7,
+        "gather": {"traffic_bytes": 662.3e6 + 1245.7e6,
+                   "k_interp_col16": {"traffic_bytes": 662.3e6, "mfma_util_pct": 24.8, "valu_busy_pct": 56.9,
+                                      "valu_insts": 2.02e8, "mfma_insts": 2.29e7, "fma_insts": 5.78e7,
                                       "lds_bank_conflicts": 3.05e7, "lds_insts": 9.63e6},
-                   "k_interp_row16": {"traffic_bytes": 1248.0e6, "valu_busy_pct": 81.9, "valu_insts": 2.73e8,
+                   "k_interp_row16": {"traffic_bytes": 1245.7e6, "valu_busy_pct": 77.9, "valu_insts": 2.38e8,
                                       "fma_insts": 1.2e8, "lds_bank_conflicts": 9.85e7, "lds_insts": 2.72e7}},
     },
     ("uniform", 1_000_000, 16, 128, "fp32", 12): {
